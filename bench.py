@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Throughput of the EA tube Monte Carlo engine on one MI355X.
+
+A step is one pass of the reference's inner loop over all samples (main.cu
+:149-228 with ptfr = houfr = mesfr = 1): a Metropolis sweep of every
+(site, replica, temperature, sample), the energy, parallel tempering, the
+Houdayer move (when the workload has it) and the per-step energy measure.
+The metric is the reference's own unit (timing.c:27-34):
+
+    spin-flip attempts/ns = steps * V * NR * NT * S / wall_ns
+
+Default workload: the OBC tube L=16, Lz=64, 4 replicas, 24 temperatures in
+[0.6, 1.6], S=31 samples, Houdayer at the lround(24/2)=12 lowest
+temperatures every step.  Inputs (spins, couplings, PT state) are resident
+on the GPU before the timed region; the host-side counter blocks and replica
+pairs of every step are generated inside it.
+
+Prints one JSON line.  --gpus above 1 is refused: this build measures one
+GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "ea-lcd-mc_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+import eatube  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # the configuration the headline metric is quoted on
+    "obc_L16_Lz64": dict(L=16, Lz=64, nr=4, nt=24, tmin=0.6, tmax=1.6,
+                         bc=eatube.OBC, cluster=True),
+    "plain_L8_Lz16": dict(L=8, Lz=16, nr=2, nt=16, tmin=0.7, tmax=1.6,
+                          bc=eatube.PBC, cluster=False),
+    "large_L48_Lz384": dict(L=48, Lz=384, nr=4, nt=32, tmin=0.5, tmax=1.6,
+                            bc=eatube.PBC, cluster=True),
+}
+SEEDS = (20240601, 20240602, 20240603, 20240604)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def b_alg_sweep(S, nr, nt):
+    """Algorithmic HBM bytes per spin-flip attempt of the sweep: own word
+    read+write, the opposite parity read once, 3 random words shared by the
+    S samples, 6 coupling words shared by NR*NT words (per 64 attempts)."""
+    return 0.375 + 0.375 / S + 0.75 / (nr * nt)
+
+
+def kernel_bytes(kind, w, S, V, ctm):
+    """algorithmic bytes of one launch of each kernel kind"""
+    nr, nt = w["nr"], w["nt"]
+    if kind == "sweep":
+        return b_alg_sweep(S, nr, nt) * V * nr * nt * S / 2.0
+    if kind == "houdayer":
+        # q read and both replicas written: 0.5 B per labelled site when the
+        # labels stay on chip; 8.5 B when they round-trip HBM
+        per_site = 0.5 if V <= 32768 else 8.5
+        return per_site * V * ctm * S * 2
+    if kind == "energy":
+        return V * nr * nt * S / 8.0 + 6 * V / 8.0 * S  # spins + couplings
+    if kind == "pt":
+        return S * nr * (2080 * 2 + nt * 4 + nt * 2 + (nt - 1) * 24)
+    return 0.0
+
+
+def parity_check(w):
+    """Two steps of two samples of the workload against the oracle."""
+    import orc
+    betas = eatube.betas_linear(w["tmin"], w["tmax"], w["nt"])
+    args = (w["L"], w["Lz"], w["nr"], w["nt"], 2)
+    e = eatube.Simulation(*args, bc=w["bc"], betas=betas, seeds=SEEDS)
+    o = orc.Sim(*args, bc=w["bc"], betas=betas, seeds=SEEDS)
+    e.steps(2, cluster=w["cluster"])
+    o.steps(2, cluster=w["cluster"])
+    ok = np.array_equal(e.spins(), o.spins()) and np.array_equal(
+        e.engine.download_perms().ravel(), o.field("wctb"))
+    o.energy()
+    ok = ok and np.array_equal(e.engine.energy().ravel(), o.field("ener"))
+    e.engine.close()
+    return ok
+
+
+def cpu_baseline(w, S, budget_s):
+    """The oracle (scalar C restatement of the reference algorithm) on one
+    host core, over a bounded number of steps of the same workload."""
+    import orc
+    betas = eatube.betas_linear(w["tmin"], w["tmax"], w["nt"])
+    o = orc.Sim(w["L"], w["Lz"], w["nr"], w["nt"], S, bc=w["bc"],
+                betas=betas, seeds=SEEDS)
+    t0 = time.perf_counter()
+    o.steps(1, cluster=w["cluster"])
+    t1 = time.perf_counter() - t0
+    n = int(max(1, min(200, budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    o.steps(n, cluster=w["cluster"])
+    dt = time.perf_counter() - t0
+    V = w["L"] * w["L"] * w["Lz"]
+    return {"value": n * V * w["nr"] * w["nt"] * S / (dt * 1e9),
+            "unit": "spin-flip attempts/ns", "cores": 1, "kind": "port",
+            "sample": "%d steps of the same workload (all %d samples), "
+                      "%.1f s on one core" % (n, S, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="obc_L16_Lz64",
+                    choices=sorted(WORKLOADS))
+    ap.add_argument("--samples", type=int, default=31)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU; runs on %d GPUs are not part of "
+              "this build" % max(args.gpus, int(os.environ.get("WORLD_SIZE",
+                                                               "1"))),
+              file=sys.stderr)
+        sys.exit(2)
+    if eatube.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    w = WORKLOADS[args.workload]
+    S = args.samples
+    V = w["L"] * w["L"] * w["Lz"]
+    betas = eatube.betas_linear(w["tmin"], w["tmax"], w["nt"])
+
+    parity = None
+    if not args.no_parity:
+        parity = parity_check(w)
+        log("parity vs oracle:", parity)
+        if not parity:
+            print(json.dumps({"metric": "spin-flip attempts/ns",
+                              "error": "bit-exact check against the oracle "
+                                       "failed; no timing reported"}))
+            sys.exit(1)
+
+    sim = eatube.Simulation(w["L"], w["Lz"], w["nr"], w["nt"], S, bc=w["bc"],
+                            betas=betas, seeds=SEEDS)
+    eng = sim.engine
+    ctm = eng.ctm
+    log("warmup", args.warmup)
+    sim.steps(args.warmup, cluster=w["cluster"])
+    eng.sync()
+    log("timed", args.steps)
+    t0 = time.perf_counter()
+    sch = sim.schedule(args.steps, cluster=w["cluster"])
+    eng.run(sch)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    attempts = args.steps * V * w["nr"] * w["nt"] * S
+    value = attempts / (wall * 1e9)
+
+    # per-kernel durations, HIP events on the engine's stream
+    eng.set_timing(True)
+    sim.steps(args.timing_steps, cluster=w["cluster"])
+    tm = eng.read_timing()
+    eng.set_timing(False)
+    kern = {}
+    for k, (ms, n) in tm.items():
+        if n:
+            b = kernel_bytes(k, w, S, V, ctm)
+            kern[k] = {"avg_us": 1e3 * ms / n, "launches": n,
+                       "share": 0.0, "GBs": b / (1e-3 * ms / n) / 1e9}
+    tot = sum(v["avg_us"] * v["launches"] for v in kern.values())
+    for v in kern.values():
+        v["share"] = v["avg_us"] * v["launches"] / tot
+    dom = max(kern, key=lambda k: kern[k]["share"])
+    ach = kern[dom]["GBs"]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": ach,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": kernel_bytes(dom, w, S, V, ctm),
+            "avg_launch_us": kern[dom]["avg_us"]}
+    sweep = kern.get("sweep")
+    out = {
+        "metric": "spin-flip attempts/ns",
+        "value": value,
+        "unit": "spin-flip attempts/ns",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded couplings and spins, reference seeding chain)",
+        "config": {"workload": args.workload, "L": w["L"], "Lz": w["Lz"],
+                   "bc": "OBC" if w["bc"] else "PBC", "replicas": w["nr"],
+                   "temperatures": w["nt"], "T_range": [w["tmin"], w["tmax"]],
+                   "samples": S, "houdayer": bool(w["cluster"]),
+                   "houdayer_betas": ctm if w["cluster"] else 0,
+                   "step": "sweep+energy+PT" + ("+Houdayer" if w["cluster"]
+                                                else "") + "+measure"},
+        "parity_vs_oracle": parity,
+        "kernels": kern,
+        "roofline": roof,
+    }
+    if sweep:
+        out["sweep_roofline"] = {"achieved": sweep["GBs"], "peak": HBM_PEAK_GBS,
+                                 "frac": sweep["GBs"] / HBM_PEAK_GBS,
+                                 "avg_launch_us": sweep["avg_us"]}
+    if not args.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(w, S, args.cpu_seconds)
+    eng.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

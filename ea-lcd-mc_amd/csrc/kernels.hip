@@ -1,0 +1,728 @@
+// kernels.hip -- gfx950 kernels of the EA tube engine.
+//
+// Layouts (per sample s): spins [2 parities][nt clones][n half sites][nr
+// replicas] of 64-bit words, bit = bx + 4 by + 16 bz of the 4x4x4 copies of
+// an MSC site (ini.c:339-383); couplings [2][n][6] words, 48 contiguous
+// bytes per half site so the nr replicas of a site read them as one
+// broadcast.  All kernels are wave64.
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace eatube {
+
+namespace {
+
+constexpr uint64_t kLcg = 3202034522624059733ULL;  // random_generator.h:133
+constexpr double kFnorm = 5.4210108624275218e-20;  // random_generator.h:92
+
+// ---------------------------------------------------------------- helpers
+
+// Philox4x32-10, Salmon et al. SC'11 (the round and key schedule of
+// main.cu:996-1043).
+__device__ __forceinline__ void philox10(uint32_t &v0, uint32_t &v1,
+                                         uint32_t &v2, uint32_t &v3,
+                                         uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const uint32_t lo0 = 0xD2511F53u * v0, hi0 = __umulhi(0xD2511F53u, v0);
+    const uint32_t lo1 = 0xCD9E8D57u * v2, hi1 = __umulhi(0xCD9E8D57u, v2);
+    v0 = hi1 ^ v1 ^ k0;
+    v1 = lo1;
+    v2 = hi0 ^ v3 ^ k1;
+    v3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// R = (m:l) < umbral[i]; the LUT sits in LDS as 256 (hi, lo) threshold
+// pairs followed by 4096 16-bit (min | max << 8) prefix windows.
+__device__ __forceinline__ bool below(const uint32_t *lut, uint32_t i,
+                                      uint32_t m, uint32_t l) {
+  const uint32_t hx = lut[2 * i], lx = lut[2 * i + 1];
+  return m == hx ? l < lx : m < hx;
+}
+
+// Byte whose bits are Bernoulli(exp(-4 beta)): the interval of the 64-bit
+// uniform among the 256 cumulative thresholds, narrowed by the 12-bit prefix
+// window and then bisected (main.cu:1063-1078).
+__device__ __forceinline__ uint32_t lut_byte(const uint32_t *lut, uint32_t m,
+                                             uint32_t l) {
+  const uint16_t *pre = reinterpret_cast<const uint16_t *>(lut + 512);
+  uint32_t lo = 255, hi = 255;
+  if (below(lut, 254, m, l)) {
+    const uint32_t w = pre[m >> 20];
+    lo = w & 255;
+    hi = w >> 8;
+  }
+  while (hi - lo > 1) {
+    const uint32_t mid = (hi + lo) >> 1;
+    if (below(lut, mid, m, l))
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return below(lut, lo, m, l) ? lo : hi;
+}
+
+// One 32-bit random word: Philox, the 64-bit uniforms (v[j] : lsb[j]), four
+// LUT bytes and a data-dependent rotation (main.cu:1045-1092).  A rotation
+// by 0 leaves the word unchanged.
+__device__ __forceinline__ uint32_t rnd_word(uint32_t x, uint32_t y, uint32_t z,
+                                             uint32_t w, uint32_t k0,
+                                             uint32_t k1, const uint32_t *lut) {
+  uint32_t v0 = x, v1 = y, v2 = z, v3 = w;
+  philox10(v0, v1, v2, v3, k0, k1);
+  const uint32_t hi = v0 ^ v2, lo = v1 ^ v3;
+  const uint32_t l0 = ((v1 << 17) | (v0 >> 15)) ^ hi ^ ((hi << 21) | (lo >> 11));
+  const uint32_t l1 = ((v0 << 17) | (v1 >> 15)) ^ lo ^ (lo << 21);
+  const uint32_t l2 = (hi << 28) | (lo >> 4);
+  const uint32_t l3 = (lo << 28) | (hi >> 4);
+  const uint32_t rot = (l0 + v1 + l2 + v3) >> 27;
+  const uint32_t bb = lut_byte(lut, v0, l0) | (lut_byte(lut, v1, l1) << 8) |
+                      (lut_byte(lut, v2, l2) << 16) |
+                      (lut_byte(lut, v3, l3) << 24);
+  return (bb << rot) | (bb >> ((32 - rot) & 31));
+}
+
+// Bit permutations bringing a neighbour word whose MSC site wrapped around
+// the sub-lattice to the copy layout of the centre word.
+constexpr uint64_t kMaskE = 0x1111111111111111ULL;
+constexpr uint64_t kMaskN = 0x000F000F000F000FULL;
+constexpr uint64_t kMaskU = 0x000000000000FFFFULL;
+constexpr uint64_t kMaskO = 0x8888888888888888ULL;
+constexpr uint64_t kMaskS = 0xF000F000F000F000ULL;
+constexpr uint64_t kMaskD = 0xFFFF000000000000ULL;
+__device__ __forceinline__ uint64_t wrap_xp(uint64_t o) {
+  return (o & kMaskE) << 3 | (o & ~kMaskE) >> 1;
+}
+__device__ __forceinline__ uint64_t wrap_yp(uint64_t o) {
+  return (o & kMaskN) << 12 | (o & ~kMaskN) >> 4;
+}
+__device__ __forceinline__ uint64_t wrap_zp(uint64_t o) {
+  return (o & kMaskU) << 48 | (o & ~kMaskU) >> 16;
+}
+__device__ __forceinline__ uint64_t wrap_xm(uint64_t o) {
+  return (o & kMaskO) >> 3 | (o & ~kMaskO) << 1;
+}
+__device__ __forceinline__ uint64_t wrap_ym(uint64_t o) {
+  return (o & kMaskS) >> 12 | (o & ~kMaskS) << 4;
+}
+__device__ __forceinline__ uint64_t wrap_zm(uint64_t o) {
+  return (o & kMaskD) >> 48 | (o & ~kMaskD) << 16;
+}
+
+// Neighbours of the parity-p site at half index h, in closed form: half
+// index of each of the six opposite-parity neighbours (+x,+y,+z,-x,-y,-z)
+// and a 6-bit mask of the directions that wrapped around the sub-lattice.
+struct Nbrs {
+  int h[6];
+  int wrap;
+  int gz;
+};
+
+__device__ __forceinline__ Nbrs neighbours(const Lattice &g, int p, int h) {
+  const int gz = h / g.hs, rem = h - gz * g.hs, by = rem / g.hl;
+  const int ax = 2 * (rem - by * g.hl) + ((p + by + gz) & 1);
+  Nbrs nb;
+  nb.gz = gz;
+  int w = 0;
+  const int row = g.ML * by + g.MS * gz;  // site index of (0, by, gz)
+  const int xp = ax + 1 == g.ML ? (w |= 1, 0) : ax + 1;
+  const int xm = ax == 0 ? (w |= 8, g.ML - 1) : ax - 1;
+  nb.h[0] = (xp + row) >> 1;
+  nb.h[3] = (xm + row) >> 1;
+  const int yp = by + 1 == g.ML ? (w |= 2, 0) : by + 1;
+  const int ym = by == 0 ? (w |= 16, g.ML - 1) : by - 1;
+  nb.h[1] = (ax + g.ML * yp + g.MS * gz) >> 1;
+  nb.h[4] = (ax + g.ML * ym + g.MS * gz) >> 1;
+  const int zp = gz + 1 == g.MLz ? (w |= 4, 0) : gz + 1;
+  const int zm = gz == 0 ? (w |= 32, g.MLz - 1) : gz - 1;
+  nb.h[2] = (ax + g.ML * by + g.MS * zp) >> 1;
+  nb.h[5] = (ax + g.ML * by + g.MS * zm) >> 1;
+  nb.wrap = w;
+  return nb;
+}
+
+// The six neighbour words of (clone row `old`, replica r), brought to the
+// centre's copy layout; with open z boundaries the z bond of the top
+// (bottom) copy block is replaced by the y+1 (even) or y-1 (odd) neighbour
+// (main.cu:1460-1472).
+__device__ __forceinline__ void load_neighbours(const Lattice &g,
+                                                const uint64_t *old, int r,
+                                                const Nbrs &nb, int parity,
+                                                uint64_t sc[6]) {
+#pragma unroll
+  for (int d = 0; d < 6; ++d) sc[d] = old[(size_t)nb.h[d] * g.nr + r];
+  if (nb.wrap) {
+    if (nb.wrap & 1) sc[0] = wrap_xp(sc[0]);
+    if (nb.wrap & 2) sc[1] = wrap_yp(sc[1]);
+    if (nb.wrap & 4) sc[2] = wrap_zp(sc[2]);
+    if (nb.wrap & 8) sc[3] = wrap_xm(sc[3]);
+    if (nb.wrap & 16) sc[4] = wrap_ym(sc[4]);
+    if (nb.wrap & 32) sc[5] = wrap_zm(sc[5]);
+  }
+  if (g.bc == EA_OBC) {
+    const uint64_t aux = parity == 0 ? sc[1] : sc[4];
+    if (nb.gz == g.MLz - 1) sc[2] = (sc[2] & ~kMaskD) | (aux & kMaskD);
+    if (nb.gz == 0) sc[5] = (sc[5] & ~kMaskU) | (aux & kMaskU);
+  }
+}
+
+// Sum of e over the lanes of this wave holding the same replica
+// (lane & (nr-1)); result valid in every lane.
+__device__ __forceinline__ int replica_sum(int e, int nr) {
+  for (int off = nr; off < 64; off <<= 1) e += __shfl_xor(e, off, 64);
+  return e;
+}
+
+// ------------------------------------------------------------------ sweep
+
+// One half-sweep of every sample.  Thread (beta, h, r) draws the three
+// random words of the reference thread tid = beta*n*nr + h*nr + r
+// (d_gen_rndbits_cuda, main.cu:1095-1150) and applies them to the clone that
+// sits at beta in each sample (d_oneMCstepBN_multisample,
+// main.cu:1385-1510): the words are shared by all samples, so they are made
+// once and never leave registers.
+__global__ void __launch_bounds__(256) k_sweep(Lattice g, SweepArgs a) {
+  __shared__ uint32_t lut[EA_LUT_BYTES / 4];
+  __shared__ uint8_t clone_at[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
+  const int beta = blockIdx.y;
+  const uint32_t *src =
+      reinterpret_cast<const uint32_t *>(a.lut + (size_t)beta * EA_LUT_BYTES);
+  for (int i = threadIdx.x; i < EA_LUT_BYTES / 4; i += blockDim.x) lut[i] = src[i];
+  for (int i = threadIdx.x; i < g.ns * g.nr; i += blockDim.x)
+    clone_at[i] = a.wctb[(size_t)i * g.nt + beta];
+  if (a.energy_mode == 1 && blockIdx.x == 0 && blockIdx.y == 0)
+    for (int i = threadIdx.x; i < g.ns * g.nr * g.nt; i += blockDim.x) a.ener[i] = 0;
+  __syncthreads();
+
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nwords = g.n * g.nr;
+  const bool valid = t < nwords;
+  const int h = valid ? t >> g.nr_shift : 0, r = t & (g.nr - 1);
+
+  uint64_t id1 = 0, id2 = 0;
+  if (valid) {
+    const uint32_t tid = (uint32_t)beta * (uint32_t)nwords + (uint32_t)t;
+    const ea_counter c = a.ctr[r];
+    const uint32_t key = a.keys[r], z = c.z + tid;
+    uint32_t w[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = rnd_word(c.x + k, c.y, z, c.w, tid, key, lut);
+    const uint64_t b0 = ((uint64_t)w[0] << 32) | w[1];
+    const uint64_t b1 = (((uint64_t)w[2] << 32) | w[3]) & b0;
+    const uint64_t b2 = (((uint64_t)w[4] << 32) | w[5]) & b1;
+    // id = id1 + 2 id2 in 0..3 with P(id >= k) = exp(-4 k beta)
+    id2 = b1;
+    id1 = (b0 ^ b1) | b2;
+  }
+  const int p = a.parity;
+  const Nbrs nb = neighbours(g, p, h);
+  const bool wave_live = (int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)) < nwords;
+
+  for (int s = 0; s < g.ns; ++s) {
+    const int clone = clone_at[s * g.nr + r];
+    int e = 0;
+    if (valid) {
+      uint64_t *base = a.spin + (size_t)s * 2 * g.half + (size_t)clone * nwords;
+      uint64_t *cur = base + (size_t)p * g.half;
+      const uint64_t *old = base + (size_t)(1 - p) * g.half;
+      const uint64_t sv = cur[(size_t)h * g.nr + r];
+      uint64_t sc[6];
+      load_neighbours(g, old, r, nb, p, sc);
+      const uint64_t *Jw = a.J + ((size_t)(2 * s + p) * g.n + h) * 6;
+      uint64_t f[6];
+#pragma unroll
+      for (int d = 0; d < 6; ++d) f[d] = sv ^ Jw[d] ^ sc[d];  // 1 = unsatisfied
+      // (j2 j1) = f0+f1+f2, (j4 j3) = f3+f4+f5; flip iff j + id >= 3
+      uint64_t k1 = f[0] ^ f[1], k2 = f[0] & f[1];
+      const uint64_t j1 = k1 ^ f[2], j2 = k2 ^ (k1 & f[2]);
+      k1 = f[3] ^ f[4];
+      k2 = f[3] & f[4];
+      const uint64_t j3 = k1 ^ f[5], j4 = k2 ^ (k1 & f[5]);
+      const uint64_t j24 = j2 | j4;
+      const uint64_t flip = (j1 & j3 & id1) | ((j24 | id2) & (j1 | j3 | id1)) |
+                            (j2 & j4) | (j24 & id2);
+      cur[(size_t)h * g.nr + r] = sv ^ flip;
+      if (a.energy_mode == 2) {
+        int u = 0;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) u += __popcll(f[d] ^ flip);
+        e = 6 * 64 - 2 * u;
+      }
+    }
+    if (a.energy_mode == 2) {
+      // every bond has exactly one odd end: the odd half-sweep sees them all
+      e = replica_sum(e, g.nr);
+      if (wave_live && (threadIdx.x & 63) < g.nr)
+        atomicAdd(a.ener + ((size_t)s * g.nr + r) * g.nt + clone, e);
+    }
+  }
+}
+
+// ----------------------------------------------------------------- energy
+
+// Energy of every (sample, replica, clone) from the even words
+// (d_computeEne, main.cu:1209-1381): sum of 384 - 2 popcount(unsatisfied).
+__global__ void __launch_bounds__(256) k_energy(Lattice g, const uint64_t *spin,
+                                                const uint64_t *J, int32_t *ener) {
+  const int clone = blockIdx.y, s = blockIdx.z;
+  const int nwords = g.n * g.nr;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < nwords;
+  const int h = valid ? t >> g.nr_shift : 0, r = t & (g.nr - 1);
+  int e = 0;
+  if (valid) {
+    const uint64_t *base = spin + (size_t)s * 2 * g.half + (size_t)clone * nwords;
+    const uint64_t sv = base[(size_t)h * g.nr + r];
+    const Nbrs nb = neighbours(g, 0, h);
+    uint64_t sc[6];
+    load_neighbours(g, base + g.half, r, nb, 0, sc);
+    const uint64_t *Jw = J + ((size_t)(2 * s) * g.n + h) * 6;
+    int u = 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
+    e = 6 * 64 - 2 * u;
+  }
+  e = replica_sum(e, g.nr);
+  if ((int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)) < nwords &&
+      (threadIdx.x & 63) < g.nr)
+    atomicAdd(ener + ((size_t)s * g.nr + r) * g.nt + clone, e);
+}
+
+// ------------------------------------------------------ parallel tempering
+
+// exp(pt) of the host's libm for pair i and energy difference d < 0: the
+// tabulated value, 0 where exp underflows.
+__device__ __forceinline__ double exact_exp(const PtArgs &a, int i, int d) {
+  // energies are sums of 384 - 2*popcount terms, so d is even and k >= 1
+  const int k = -d / 2;
+  if (k < 1) return exp(a.nd[i] * (double)d);
+  return k <= a.len[i] ? a.tab[a.off[i] + k - 1] : 0.0;
+}
+
+// Parallel_Tempering (upd.c:3-48), one wave per (sample, replica) chain.
+// The chain's Parisi-Rapuano state is staged in LDS; lane 0 walks the beta
+// pairs in order.  The swap decision compares FNORM*final with exp(pt): the
+// device exp decides unless the two are within 2^-40 relative, where the
+// host-libm table decides, so every decision equals the reference's.  The
+// statistics are then added by one lane per pair with the tabulated values.
+__global__ void __launch_bounds__(64) k_pt(Lattice g, PtArgs a) {
+  __shared__ uint64_t ira[256];
+  __shared__ int32_t E[EA_MAX_BETAS];
+  __shared__ uint8_t perm[EA_MAX_BETAS + 1];
+  __shared__ int32_t dE[EA_MAX_BETAS];
+  __shared__ uint8_t took[EA_MAX_BETAS];
+  const int chain = blockIdx.x, nt = g.nt;
+  ea_hq_state *st = a.st + chain;
+  for (int i = threadIdx.x; i < 256; i += 64) ira[i] = st->ira[i];
+  for (int i = threadIdx.x; i < nt; i += 64) {
+    E[i] = a.ener[(size_t)chain * nt + i];
+    perm[i] = a.wctb[(size_t)chain * nt + i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint8_t ip = st->ip, ip1 = st->ip1, ip2 = st->ip2, ip3 = st->ip3;
+    uint64_t zseed = st->zseed, PR = st->PR, fin = st->final_;
+    for (int i = 0; i + 1 < nt; ++i) {
+      ira[ip] = ira[ip1] + ira[ip2];
+      PR = ira[ip] ^ ira[ip3];
+      zseed = zseed * kLcg + 1;
+      fin = zseed + PR;
+      ++ip;
+      ++ip1;
+      ++ip2;
+      ++ip3;
+      const int d = E[perm[i + 1]] - E[perm[i]];
+      const double pt = a.nd[i] * (double)d;
+      bool take = true;
+      if (!(pt >= 0)) {
+        const double u = kFnorm * (double)fin;
+        const double ex = exp(pt);
+        if (u == 0.0 || fabs(u - ex) <= ex * 0x1p-40)
+          take = u < exact_exp(a, i, d);
+        else
+          take = u < ex;
+      }
+      dE[i] = d;
+      took[i] = take;
+      if (take) {
+        const uint8_t tmp = perm[i];
+        perm[i] = perm[i + 1];
+        perm[i + 1] = tmp;
+      }
+    }
+    st->ip = ip;
+    st->ip1 = ip1;
+    st->ip2 = ip2;
+    st->ip3 = ip3;
+    st->zseed = zseed;
+    st->PR = PR;
+    st->final_ = fin;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 256; i += 64) st->ira[i] = ira[i];
+  for (int i = threadIdx.x; i < nt; i += 64) a.wctb[(size_t)chain * nt + i] = perm[i];
+  for (int i = threadIdx.x; i + 1 < nt; i += 64) {
+    const size_t k = (size_t)chain * (nt - 1) + i;
+    const double pt = a.nd[i] * (double)dE[i];
+    a.att[k] += 1.0;
+    a.acc[k] += pt >= 0 ? 1.0 : exact_exp(a, i, dE[i]);
+    if (took[i]) a.raw[k] += 1.0;
+  }
+}
+
+// --------------------------------------------------------------- measures
+
+// Per-beta energy sums and the beta-of-clone history (main.cu:208-228).
+__global__ void k_measure(Lattice g, MeasureArgs a) {
+  const int chain = blockIdx.x, nt = g.nt;
+  for (int b = threadIdx.x; b < nt; b += blockDim.x) {
+    const int c = a.wctb[(size_t)chain * nt + b];
+    a.esum[(size_t)chain * nt + b] += (double)a.ener[(size_t)chain * nt + c];
+    if (a.row >= 0)
+      a.hist[((size_t)a.row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_f64(double *p, double v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// --------------------------------------------------------------- Houdayer
+
+// Cluster coin: U < 1/2 with U from Philox4x32-10 at counter
+// (idx_lo, idx_hi, 0x486f7564, 0) and key (seed_lo, seed_hi).  The index
+// scheme follows the reference's cuRAND use (HoudayerXEA.cu:174-195,373):
+// idx = offset + s*V*NT + beta*V + site.
+__device__ __forceinline__ bool coin(uint64_t seed, uint64_t idx) {
+  uint32_t v0 = (uint32_t)idx, v1 = (uint32_t)(idx >> 32), v2 = 0x486f7564u, v3 = 0;
+  philox10(v0, v1, v2, v3, (uint32_t)seed, (uint32_t)(seed >> 32));
+  return v0 < 0x80000000u;
+}
+
+// Word offsets of replicas r1, r2 of the instance (group, sample, beta).
+struct Pair {
+  size_t o1, o2;
+};
+__device__ __forceinline__ Pair pair_words(const Lattice &g, const HouArgs &a,
+                                           int s, int grp, int beta) {
+  const int r1 = a.pairs[2 * grp], r2 = a.pairs[2 * grp + 1];
+  const int c1 = a.wctb[((size_t)s * g.nr + r1) * g.nt + beta];
+  const int c2 = a.wctb[((size_t)s * g.nr + r2) * g.nt + beta];
+  const size_t sb = (size_t)s * 2 * g.half;
+  Pair q;
+  q.o1 = sb + (size_t)c1 * g.n * g.nr + r1;
+  q.o2 = sb + (size_t)c2 * g.n * g.nr + r2;
+  return q;
+}
+
+// Linear site index (x + L y + L^2 z) of copy 0 of the parity-p site at half
+// index h; copy `bit` adds copy_offset(bit).
+__device__ __forceinline__ int site_base(const Lattice &g, int p, int h) {
+  const int gz = h / g.hs, rem = h - gz * g.hs, by = rem / g.hl;
+  const int ax = 2 * (rem - by * g.hl) + ((p + by + gz) & 1);
+  return ax + g.L * (by + g.L * gz);
+}
+__device__ __forceinline__ int copy_offset(const Lattice &g, int bit) {
+  return (bit & 3) * g.ML + g.L * (((bit >> 2) & 3) * g.ML + g.L * ((bit >> 4) * g.MLz));
+}
+
+__device__ __forceinline__ uint32_t find_root(const uint32_t *lab, uint32_t a) {
+  const volatile uint32_t *v = lab;
+  uint32_t p = v[a];
+  while (p != a) {
+    a = p;
+    p = v[a];
+  }
+  return a;
+}
+
+// Link the trees of a and b, the larger root under the smaller one; a link
+// displaced by a concurrent union is re-linked (Komura, CPC 194 (2015) 54).
+__device__ __forceinline__ void unite(uint32_t *lab, uint32_t a, uint32_t b) {
+  a = find_root(lab, a);
+  b = find_root(lab, b);
+  while (a != b) {
+    if (a < b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    const uint32_t old = atomicMin(lab + a, b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+__device__ __forceinline__ uint32_t qbit(const uint32_t *qb, int i) {
+  return (qb[i >> 5] >> (i & 31)) & 1u;
+}
+
+// Union along +x, +y (periodic) and +z of local site i of a slab of planes
+// [z0, z1) whose labels start at linear site z0*L*L.  z bonds leaving the
+// slab are skipped unless the slab is the whole periodic tube.
+__device__ __forceinline__ void slab_unions(const Lattice &g, uint32_t *lab,
+                                            const uint32_t *qb, int i, int planes,
+                                            bool wrap_z) {
+  const int L = g.L, S = L * L;
+  const int x = i % L, yz = i / L, y = yz % L, z = yz / L;
+  const uint32_t qi = qbit(qb, i);
+  int j = x + 1 == L ? i - (L - 1) : i + 1;
+  if (qbit(qb, j) == qi) unite(lab, i, j);
+  j = y + 1 == L ? i - L * (L - 1) : i + L;
+  if (qbit(qb, j) == qi) unite(lab, i, j);
+  if (z + 1 < planes) {
+    j = i + S;
+    if (qbit(qb, j) == qi) unite(lab, i, j);
+  } else if (wrap_z) {
+    j = i - S * (planes - 1);
+    if (qbit(qb, j) == qi) unite(lab, i, j);
+  }
+}
+
+// Whole-tube Houdayer move of one instance (group, sample, beta < ctm) in a
+// single workgroup: overlap q = s1 ^ s2 of replicas r1, r2 at beta
+// (d_computeoverlap), clusters of equal q labelled by union-find in LDS
+// (init_YK_custom/ReduceLabels/analysis, HoudayerXEA.cu:77-121,233-354),
+// label = minimum linear site index, one coin per cluster root
+// (spin_select, :358-374), then q' = q ^ coin(label) and s1 ^= q', s2 ^= q'
+// (spin_flip_YK_custom + d_pack + d_useoverlap).
+template <int MAXV>
+__global__ void __launch_bounds__(1024) k_hou_whole(Lattice g, HouArgs a) {
+  __shared__ uint32_t lab[MAXV];
+  __shared__ uint32_t qb[MAXV / 32];
+  __shared__ uint32_t cb[MAXV / 32];
+  __shared__ int offs[64];
+  const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
+  const int V = g.V, T = blockDim.x, tid = threadIdx.x;
+  const Pair pw = pair_words(g, a, s, grp, beta);
+  for (int i = tid; i < V / 32; i += T) {
+    qb[i] = 0;
+    cb[i] = 0;
+  }
+  if (tid < 64) offs[tid] = copy_offset(g, tid);
+  for (int i = tid; i < V; i += T) lab[i] = i;
+  __syncthreads();
+  for (int w = tid; w < 2 * g.n; w += T) {
+    const int p = w >= g.n, h = w - p * g.n;
+    const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+    uint64_t q = a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
+    const int base = site_base(g, p, h);
+    while (q) {
+      const int b = __ffsll((unsigned long long)q) - 1;
+      q &= q - 1;
+      const int site = base + offs[b];
+      atomicOr(qb + (site >> 5), 1u << (site & 31));
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < V; i += T) slab_unions(g, lab, qb, i, g.Lz, g.bc == EA_PBC);
+  __syncthreads();
+  for (int i = tid; i < V; i += T) lab[i] = find_root(lab, i);
+  __syncthreads();
+  const uint64_t cidx = a.offset0 + (uint64_t)grp * V * g.nt * g.ns +
+                        (uint64_t)s * V * g.nt + (uint64_t)beta * V;
+  for (int i = tid; i < V; i += T)
+    if (lab[i] == (uint32_t)i && coin(a.seed, cidx + i))
+      atomicOr(cb + (i >> 5), 1u << (i & 31));
+  __syncthreads();
+  if (a.labels_out) {
+    uint32_t *out = a.labels_out + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
+    for (int i = tid; i < V; i += T) out[i] = lab[i];
+  }
+  for (int w = tid; w < 2 * g.n; w += T) {
+    const int p = w >= g.n, h = w - p * g.n;
+    const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+    const int base = site_base(g, p, h);
+    uint64_t m = 0;
+    for (int b = 0; b < 64; ++b) {
+      const uint32_t root = lab[base + offs[b]];
+      m |= (uint64_t)((cb[root >> 5] >> (root & 31)) & 1u) << b;
+    }
+    const uint64_t x1 = a.spin[pw.o1 + o], x2 = a.spin[pw.o2 + o];
+    const uint64_t qn = x1 ^ x2 ^ m;
+    a.spin[pw.o1 + o] = x1 ^ qn;
+    a.spin[pw.o2 + o] = x2 ^ qn;
+  }
+}
+
+// Tiled Houdayer for tubes larger than one workgroup's LDS.  Pass 1 labels
+// each slab of `planes` z-planes in LDS and stores global labels; pass 2
+// unites across slab boundaries in global memory; pass 3 flattens and draws
+// the root coins; pass 4 applies the move.
+__device__ __forceinline__ uint32_t q_site(const Lattice &g, const HouArgs &a,
+                                           const Pair &pw, int site) {
+  const int x = site % g.L, yz = site / g.L, y = yz % g.L, z = yz / g.L;
+  const int p = (x ^ y ^ z) & 1;
+  const int bit = x / g.ML + 4 * (y / g.ML) + 16 * (z / g.MLz);
+  const int h = ((x % g.ML) + g.ML * (y % g.ML) + g.MS * (z % g.MLz)) >> 1;
+  const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+  return (uint32_t)(((a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o]) >> bit) & 1);
+}
+
+__device__ __forceinline__ size_t instance(const Lattice &g, int grp, int s,
+                                           int beta) {
+  return ((size_t)grp * g.ns + s) * g.ctm + beta;
+}
+
+template <int MAXV>
+__global__ void __launch_bounds__(1024) k_hou_slab(Lattice g, HouArgs a) {
+  __shared__ uint32_t lab[MAXV];
+  __shared__ uint32_t qb[MAXV / 32];
+  const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
+  const int S = g.L * g.L, z0 = blockIdx.x * a.planes;
+  const int planes = min(a.planes, g.Lz - z0), nsite = planes * S;
+  const int T = blockDim.x, tid = threadIdx.x;
+  const Pair pw = pair_words(g, a, s, grp, beta);
+  const size_t ib = instance(g, grp, s, beta);
+  const int gbase = z0 * S;
+  for (int c = tid; c < nsite / 32; c += T) {
+    uint32_t word = 0;
+    for (int k = 0; k < 32; ++k) word |= q_site(g, a, pw, gbase + 32 * c + k) << k;
+    qb[c] = word;
+    a.cbits[ib * (g.V / 32) + gbase / 32 + c] = 0;
+  }
+  for (int i = tid; i < nsite; i += T) lab[i] = i;
+  __syncthreads();
+  for (int i = tid; i < nsite; i += T) slab_unions(g, lab, qb, i, planes, false);
+  __syncthreads();
+  uint32_t *gl = a.lab + ib * g.V + gbase;
+  for (int i = tid; i < nsite; i += T) gl[i] = gbase + find_root(lab, i);
+}
+
+// z bonds between plane z0-1 (or Lz-1 for the periodic wrap) and plane z0,
+// the first plane of slab blockIdx.x.
+__global__ void __launch_bounds__(256) k_hou_seam(Lattice g, HouArgs a) {
+  const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
+  const int S = g.L * g.L, z0 = blockIdx.x * a.planes;
+  int zl = z0 - 1;
+  if (z0 == 0) {
+    if (g.bc != EA_PBC) return;
+    zl = g.Lz - 1;
+  }
+  const Pair pw = pair_words(g, a, s, grp, beta);
+  uint32_t *gl = a.lab + instance(g, grp, s, beta) * g.V;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    const int u = z0 * S + i, d = zl * S + i;
+    if (q_site(g, a, pw, u) == q_site(g, a, pw, d)) unite(gl, u, d);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_hou_flatten(Lattice g, HouArgs a) {
+  const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
+  const size_t ib = instance(g, grp, s, beta);
+  uint32_t *gl = a.lab + ib * g.V;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.V) return;
+  const uint32_t root = find_root(gl, i);
+  gl[i] = root;
+  if (a.labels_out) a.labels_out[ib * g.V + i] = root;
+  const uint64_t cidx = a.offset0 + (uint64_t)grp * g.V * g.nt * g.ns +
+                        (uint64_t)s * g.V * g.nt + (uint64_t)beta * g.V;
+  if (root == (uint32_t)i && coin(a.seed, cidx + i))
+    atomicOr(a.cbits + ib * (g.V / 32) + (i >> 5), 1u << (i & 31));
+}
+
+__global__ void __launch_bounds__(256) k_hou_apply(Lattice g, HouArgs a) {
+  const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= 2 * g.n) return;
+  const size_t ib = instance(g, grp, s, beta);
+  const uint32_t *gl = a.lab + ib * g.V;
+  const uint32_t *cb = a.cbits + ib * (g.V / 32);
+  const Pair pw = pair_words(g, a, s, grp, beta);
+  const int p = w >= g.n, h = w - p * g.n;
+  const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+  const int base = site_base(g, p, h);
+  uint64_t m = 0;
+  for (int b = 0; b < 64; ++b) {
+    const uint32_t root = gl[base + copy_offset(g, b)];
+    m |= (uint64_t)((cb[root >> 5] >> (root & 31)) & 1u) << b;
+  }
+  const uint64_t x1 = a.spin[pw.o1 + o], x2 = a.spin[pw.o2 + o];
+  const uint64_t qn = x1 ^ x2 ^ m;
+  a.spin[pw.o1 + o] = x1 ^ qn;
+  a.spin[pw.o2 + o] = x2 ^ qn;
+}
+
+inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+
+hipError_t launch_sweep(const Lattice &g, const SweepArgs &a, hipStream_t s) {
+  dim3 grid(blocks_for((long long)g.n * g.nr, 256), g.nt);
+  hipLaunchKernelGGL(k_sweep, grid, dim3(256), 0, s, g, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(const Lattice &g, const uint64_t *spin, const uint64_t *J,
+                         int32_t *ener, hipStream_t s) {
+  hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t *>(ener), 0,
+                                 (size_t)g.ns * g.nr * g.nt, s);
+  if (e != hipSuccess) return e;
+  dim3 grid(blocks_for((long long)g.n * g.nr, 256), g.nt, g.ns);
+  hipLaunchKernelGGL(k_energy, grid, dim3(256), 0, s, g, spin, J, ener);
+  return hipGetLastError();
+}
+
+hipError_t launch_pt(const Lattice &g, const PtArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_pt, dim3(g.ns * g.nr), dim3(64), 0, s, g, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_houdayer(const Lattice &g, const HouArgs &a, hipStream_t s) {
+  if (g.ctm <= 0) return hipSuccess;
+  if (g.V <= 16384) {
+    hipLaunchKernelGGL(k_hou_whole<16384>, dim3(1, g.ctm, 2 * g.ns), dim3(1024), 0, s,
+                       g, a);
+    return hipGetLastError();
+  }
+  if (g.V <= kHouWholeMax) {
+    hipLaunchKernelGGL(k_hou_whole<kHouWholeMax>, dim3(1, g.ctm, 2 * g.ns), dim3(1024),
+                       0, s, g, a);
+    return hipGetLastError();
+  }
+  const int slabs = (g.Lz + a.planes - 1) / a.planes;
+  hipLaunchKernelGGL(k_hou_slab<kHouWholeMax>, dim3(slabs, g.ctm, 2 * g.ns), dim3(1024),
+                     0, s, g, a);
+  hipLaunchKernelGGL(k_hou_seam, dim3(slabs, g.ctm, 2 * g.ns), dim3(256), 0, s, g, a);
+  hipLaunchKernelGGL(k_hou_flatten, dim3(blocks_for(g.V, 256), g.ctm, 2 * g.ns),
+                     dim3(256), 0, s, g, a);
+  hipLaunchKernelGGL(k_hou_apply, dim3(blocks_for(2LL * g.n, 256), g.ctm, 2 * g.ns),
+                     dim3(256), 0, s, g, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_measure(const Lattice &g, const MeasureArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_measure, dim3(g.ns * g.nr), dim3(64), 0, s, g, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  const unsigned b = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(b), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  const unsigned b = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_f64, dim3(b), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace eatube

@@ -1,0 +1,78 @@
+// kernels.hpp -- launch interface of the gfx950 kernels (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "eatube.h"
+
+namespace eatube {
+
+// Geometry and sizes, passed by value to every kernel.
+struct Lattice {
+  int L, Lz, ML, MLz, MS;  // MSC sub-lattice sides (L/4, Lz/4), MS = ML*ML
+  int n, V;                // half MSC volume V/128, sites
+  int hl, hs;              // ML/2, MS/2: strides of the half index
+  int nr, nt, ns, nr_shift, bc, ctm;
+  int64_t half;            // words of one parity array of one sample
+};
+
+// One half-sweep.  Counters and keys travel by value, as the reference's
+// s_time / s_keys kernel arguments do (main.cu:299-305).
+struct SweepArgs {
+  uint64_t *spin;        // [ns][2][nt][n][nr]
+  const uint64_t *J;     // [ns][2][n][6]
+  const uint8_t *wctb;   // [ns][nr][nt] clone at beta
+  const uint8_t *lut;    // [nt][EA_LUT_BYTES]
+  int32_t *ener;         // [ns][nr][nt] by clone
+  int parity;
+  int energy_mode;       // 0 none, 1 zero ener[], 2 accumulate ener[]
+  ea_counter ctr[EA_MAX_REPLICAS];
+  uint32_t keys[EA_MAX_REPLICAS];
+};
+
+struct PtArgs {
+  ea_hq_state *st;       // [ns][nr]
+  const int32_t *ener;   // [ns][nr][nt] by clone
+  uint8_t *wctb;         // [ns][nr][nt]
+  double *acc, *att, *raw;  // [ns][nr][nt-1]
+  const double *nd;      // [nt-1]  -(beta[i+1]-beta[i])
+  const double *tab;     // exp tables (host glibc values)
+  const int32_t *off, *len;
+};
+
+struct HouArgs {
+  uint64_t *spin;
+  const uint8_t *wctb;
+  int32_t pairs[4];
+  uint64_t seed, offset0;  // coin index of (group 0, sample 0, beta 0, site 0)
+  uint32_t *lab;         // [2][ns][ctm][V] global labels (tiled path)
+  uint32_t *cbits;       // [2][ns][ctm][V/32] root coins (tiled path)
+  uint32_t *labels_out;  // optional copy of the final labels
+  int planes;            // planes per tile (tiled path)
+};
+
+struct MeasureArgs {
+  const int32_t *ener;
+  const uint8_t *wctb;
+  double *esum;          // [ns][nr][nt] by beta
+  uint8_t *hist;         // [rows][ns][nr][nt] beta of clone
+  int row;               // history row to write, < 0 for none
+};
+
+hipError_t launch_sweep(const Lattice &g, const SweepArgs &a, hipStream_t s);
+hipError_t launch_energy(const Lattice &g, const uint64_t *spin,
+                         const uint64_t *J, int32_t *ener, hipStream_t s);
+hipError_t launch_pt(const Lattice &g, const PtArgs &a, hipStream_t s);
+hipError_t launch_houdayer(const Lattice &g, const HouArgs &a, hipStream_t s);
+hipError_t launch_measure(const Lattice &g, const MeasureArgs &a,
+                          hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
+
+// Largest lattice labelled entirely in one workgroup's LDS.
+constexpr int kHouWholeMax = 32768;
+
+}  // namespace eatube

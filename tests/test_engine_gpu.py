@@ -1,0 +1,256 @@
+"""Parity of the HIP engine with the CPU oracle (bit-exact: spins, integer
+energies, PT permutations, PT generator states and statistics, cluster
+labels), through the C ABI, on seeded inputs; and size-independent
+properties at the full workload sizes."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import eatube
+import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def pair(L, Lz, nr, nt, ns, bc=0, variant=0, mattis=0, seeds=(3, 5, 7, 9),
+         job=0, tmin=0.6, tmax=1.6, ctm=None):
+    betas = orc.betas_linear(tmin, tmax, nt)
+    say("case L=%d Lz=%d nr=%d nt=%d ns=%d bc=%d variant=%d mattis=%d" %
+        (L, Lz, nr, nt, ns, bc, variant, mattis))
+    o = orc.Sim(L, Lz, nr, nt, ns, bc=bc, variant=variant, mattis=mattis,
+                seeds=seeds, job_id=job, betas=betas, clonetbm=ctm)
+    e = eatube.Simulation(L, Lz, nr, nt, ns, bc=bc, betas=betas, seeds=seeds,
+                          job_id=job, variant=variant, mattis=mattis,
+                          clone_thr=ctm)
+    return o, e
+
+
+def assert_same_state(o, e, what=""):
+    sp = e.spins()
+    osp = o.spins()
+    for i in range(o.ns):
+        if not np.array_equal(sp[i], osp[i]):
+            bad = np.argwhere(sp[i] != osp[i])[0]
+            raise AssertionError("%s spins differ: sample %d at %s" %
+                                 (what, i, bad))
+    assert np.array_equal(e.engine.download_perms().ravel(),
+                          o.field("wctb")), what + " perms"
+    st, acc, att, raw = e.engine.download_pt_state()
+    assert bytes(st) == bytes(o.field("pt")), what + " PT generators"
+    assert np.array_equal(acc.ravel(), o.field("acc")), what + " acceptance"
+    assert np.array_equal(att.ravel(), o.field("att")), what + " attempts"
+    assert np.array_equal(raw.ravel(), o.field("raw")), what + " raw"
+    o.energy()
+    assert np.array_equal(e.engine.energy().ravel(), o.field("ener")), \
+        what + " energies"
+
+
+def test_device_present():
+    assert eatube.device_count() >= 1
+
+
+@pytest.mark.parametrize("case", [
+    dict(L=8, Lz=16, nr=2, nt=16, ns=3),
+    dict(L=8, Lz=16, nr=2, nt=5, ns=2, bc=1),
+    dict(L=16, Lz=8, nr=4, nt=3, ns=2, bc=1, variant=2),
+    dict(L=8, Lz=24, nr=8, nt=3, ns=2, bc=1, variant=1),
+    dict(L=24, Lz=16, nr=4, nt=4, ns=2),
+    dict(L=16, Lz=16, nr=16, nt=2, ns=1, bc=1),
+])
+def test_sweeps_match_oracle(case):
+    o, e = pair(**case)
+    nr = case["nr"]
+    for k in (1, 3):
+        blocks = e.counters(2 * k)
+        e.engine.sweep(blocks)
+        o.sweeps(k)
+        sp, osp = e.spins(), o.spins()
+        assert np.array_equal(sp, osp), "after %d more sweeps" % k
+    # the energies left by the fused odd half-sweep equal a fresh MSC energy
+    o.energy()
+    assert np.array_equal(e.engine.energy().ravel(), o.field("ener"))
+    assert nr == e.nr
+
+
+def test_plain_small_config_with_pt():
+    """L=8 Lz=16 PBC, 2 replicas, 16 temperatures in [0.7,1.6], PT every
+    sweep, no cluster moves (the plain small configuration, shortened)."""
+    o, e = pair(8, 16, 2, 16, 31, bc=0, tmin=0.7, tmax=1.6,
+                seeds=(101, 202, 303, 404), job=3)
+    for chunk in (1, 4, 20):
+        e.steps(chunk, cluster=False)
+        o.steps(chunk, cluster=False)
+        assert_same_state(o, e, "after %d steps" % chunk)
+    es, _ = e.engine.read_measures()
+    assert np.all(np.isfinite(es))
+
+
+@pytest.mark.parametrize("case", [
+    dict(L=8, Lz=16, nr=4, nt=6, ns=3, bc=1),
+    dict(L=16, Lz=16, nr=4, nt=8, ns=2, bc=0),
+    dict(L=8, Lz=24, nr=8, nt=4, ns=2, bc=1, variant=1),
+    dict(L=16, Lz=64, nr=4, nt=24, ns=2, bc=1, tmin=0.6, tmax=1.6),
+])
+def test_steps_with_houdayer_match_oracle(case):
+    o, e = pair(**case)
+    o.keep_labels()
+    e.engine.keep_labels()
+    for chunk in (1, 3):
+        e.steps(chunk, cluster=True)
+        o.steps(chunk, cluster=True)
+        assert_same_state(o, e, "after %d steps" % chunk)
+        lab = e.engine.download_labels().ravel()
+        assert np.array_equal(lab, o.field("last_labels")), "labels"
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_tiled_houdayer_matches_oracle(bc):
+    """V = 16*16*136 exceeds one workgroup's LDS: slab labelling + merge."""
+    o, e = pair(16, 136, 4, 4, 2, bc=bc, seeds=(21, 22, 23, 24))
+    o.keep_labels()
+    e.engine.keep_labels()
+    e.steps(2, cluster=True)
+    o.steps(2, cluster=True)
+    assert_same_state(o, e, "tiled")
+    assert np.array_equal(e.engine.download_labels().ravel(),
+                          o.field("last_labels"))
+
+
+def test_mattis_ground_state_energy():
+    o, e = pair(8, 16, 4, 4, 2, bc=0, mattis=1)
+    E = e.engine.energy()
+    assert np.all(E[:, 0, 0] == 3 * 8 * 8 * 16)
+
+
+def test_samples_are_independent_and_share_random_words():
+    """Samples differ only by their data: a sample uploaded twice evolves
+    identically in both slots, at any slot."""
+    L, Lz, nr, nt, ns = 16, 32, 4, 8, 31
+    betas = orc.betas_linear(0.6, 1.6, nt)
+    e = eatube.Simulation(L, Lz, nr, nt, ns, bc=1, betas=betas,
+                          seeds=(8, 9, 10, 11))
+    ev, od = e.engine.download_spins(4)
+    ev = ev.copy()
+    od = od.copy()
+    J = eatube.init_sample(L, Lz, nr, nt, 1, 0, 0, int(e.seeds_J[4]),
+                           int(e.seeds_u[4]))[2]
+    e.engine.upload_sample(30, ev, od, J)
+    w = e.engine.download_perms()
+    w[30] = w[4]
+    e.engine.upload_perms(w)
+    st, acc, att, raw = e.engine.download_pt_state()
+    st2 = (eatube.HQ * len(st))()
+    C.memmove(st2, st, C.sizeof(st))
+    C.memmove(C.byref(st2, 30 * nr * C.sizeof(eatube.HQ)),
+              C.byref(st, 4 * nr * C.sizeof(eatube.HQ)), nr * C.sizeof(eatube.HQ))
+    e.engine.upload_pt_state(st2)
+    # cluster coins are indexed by sample slot, so only sweeps and PT here
+    e.steps(5, cluster=False)
+    sp = e.spins()
+    assert np.array_equal(sp[4], sp[30])
+    assert not np.array_equal(sp[4], sp[5])
+
+
+# ------------------------------------------------ full-size properties
+
+def full_size_sim(L=48, Lz=384, nr=4, nt=32, ns=31, bc=0):
+    betas = orc.betas_linear(0.5, 1.6, nt)
+    return eatube.Simulation(L, Lz, nr, nt, ns, bc=bc, betas=betas,
+                             seeds=(48, 384, 4, 32))
+
+
+def test_full_size_one_sweep_bit_exact():
+    """L=48 Lz=384 PBC, 4 replicas, 32 temperatures: one sweep of two
+    samples against the oracle (the random thread index passes 2^19 here)."""
+    o, e = pair(48, 384, 4, 32, 2, bc=0, tmin=0.5, tmax=1.6,
+                seeds=(48, 384, 4, 32))
+    e.engine.sweep(e.counters(2))
+    o.sweeps(1)
+    assert np.array_equal(e.spins(), o.spins())
+    o.energy()
+    assert np.array_equal(e.engine.energy().ravel(), o.field("ener"))
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_full_size_houdayer_involution_and_conservation(bc):
+    """Large tube: the cluster move conserves E(r1)+E(r2) at every beta,
+    leaves other replicas and betas above the threshold alone, and applied
+    twice with the same coins restores the configuration (the reference's
+    TEST_HOUDAYER_ENERGY / TEST_HOUDAYER_CONF checks, main.cu:454-526)."""
+    e = full_size_sim(bc=bc, ns=8)
+    eng = e.engine
+    e.steps(2, cluster=False)
+    say("steps done")
+    before = e.spins().copy()
+    E0 = eng.energy()
+    w = eng.download_perms()
+    pairs = np.array([1, 0, 3, 2], np.int32)
+    eng.keep_labels()
+    off = eng.houdayer(pairs, 777, 12345)
+    after = e.spins().copy()
+    E1 = eng.energy()
+    ctm = eng.ctm
+    for s in range(e.ns):
+        for grp in (0, 1):
+            r1, r2 = pairs[2 * grp], pairs[2 * grp + 1]
+            for b in range(e.nt):
+                c1, c2 = w[s, r1, b], w[s, r2, b]
+                assert E0[s, r1, c1] + E0[s, r2, c2] == \
+                    E1[s, r1, c1] + E1[s, r2, c2]
+                if b >= ctm:
+                    assert E0[s, r1, c1] == E1[s, r1, c1]
+    assert not np.array_equal(before, after)
+    lab = eng.download_labels()
+    eng.houdayer(pairs, 777, 12345)
+    again = e.spins()
+    assert np.array_equal(again, before)
+    assert off == 12345 + 2 * e.V * e.nt * e.ns
+    # labels of a few instances against the oracle's breadth-first search
+    L, Lz = e.L, e.Lz
+    for (grp, s, b) in [(0, 0, 0), (1, 7, ctm - 1), (0, 5, ctm // 2)]:
+        r1, r2 = pairs[2 * grp], pairs[2 * grp + 1]
+        c1, c2 = w[s, r1, b], w[s, r2, b]
+        u1 = eatube.unpack_lattice(L, Lz, e.nr, e.nt, before[s, 0].ravel(),
+                                   before[s, 1].ravel(), r1, c1)
+        u2 = eatube.unpack_lattice(L, Lz, e.nr, e.nt, before[s, 0].ravel(),
+                                   before[s, 1].ravel(), r2, c2)
+        want = orc.labels(L, Lz, bc, u1 ^ u2)
+        assert np.array_equal(lab[grp, s, b], want), (grp, s, b)
+
+
+def test_full_size_pt_matches_oracle_pt():
+    """Device PT at 32 temperatures and 124 chains against upd.c's
+    restatement fed the same energies and generator states."""
+    e = full_size_sim(L=16, Lz=64, nt=32)
+    eng = e.engine
+    e.steps(3, cluster=False)
+    E = eng.energy()
+    st, acc, att, raw = eng.download_pt_state()
+    w = eng.download_perms()
+    eng.pt()
+    w2 = eng.download_perms()
+    st2, acc2, att2, raw2 = eng.download_pt_state()
+    ost = (orc.HQ * len(st)).from_buffer_copy(bytes(st))
+    betas = np.ascontiguousarray(e.engine.betas)
+    for ch in range(e.ns * e.nr):
+        s, r = divmod(ch, e.nr)
+        a = [np.ascontiguousarray(x[s, r]) for x in (acc, att, raw)]
+        wc = np.ascontiguousarray(w[s, r])
+        wb = np.zeros(e.nt, np.uint8)
+        En = np.ascontiguousarray(E[s, r])
+        orc.lib().orc_pt(C.byref(ost[ch]), e.nt, orc.ptr(betas, C.c_double),
+                         orc.ptr(wc, C.c_uint8), orc.ptr(wb, C.c_uint8),
+                         orc.ptr(En, C.c_int32), orc.ptr(a[0], C.c_double),
+                         orc.ptr(a[1], C.c_double), orc.ptr(a[2], C.c_double))
+        assert np.array_equal(wc, w2[s, r])
+        assert np.array_equal(a[0], acc2[s, r])
+        assert np.array_equal(a[1], att2[s, r])
+        assert np.array_equal(a[2], raw2[s, r])
+    assert bytes(ost) == bytes(st2)

@@ -1,0 +1,194 @@
+"""Host side of the engine library (no GPU): the library loads and exports
+every symbol of include/eatube.h, and its seeding, generators, sample
+initialisation, counter blocks, LUT and replica-pair draws equal the
+oracle's (itself pinned against the reference's host C) and the reference
+fixtures."""
+import ctypes as C
+import hashlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import eatube
+import orc
+from conftest import GOLDEN, ROOT, Golden
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "eatube.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ea_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    syms = header_symbols()
+    assert len(syms) >= 25
+    L = eatube.lib()
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(eatube.EXPORTS) == syms
+
+
+def test_errors_are_reported_not_fatal():
+    # an invalid configuration fails with a message, without a GPU
+    c = eatube.Config(12, 16, 4, 8, 1, 0, 4, 0, None, None)
+    h = C.c_void_p()
+    rc = eatube.lib().ea_create(C.byref(c), C.byref(h))
+    assert rc == -1
+    assert b"multiples of 8" in eatube.lib().ea_last_error()
+
+
+@pytest.mark.parametrize("seed", [1, 12345, 0xDEADBEEFCAFE, 2**63 + 5])
+def test_seed_lists_and_hq(seed):
+    assert np.array_equal(eatube.seeds_from_one(seed, 32),
+                          orc.seeds_from_one(seed, 32))
+    a, b = eatube.hq_init(seed), orc.hq_init(seed)
+    assert bytes(a) == bytes(b)
+    for _ in range(300):
+        assert eatube.lib().ea_hq_next(C.byref(a)) == \
+            orc.lib().orc_hq_next(C.byref(b))
+
+
+def test_generators_and_counters_match_oracle():
+    nr, job = 4, 77
+    prc_e, xo_e = (eatube.HQ * nr)(), (eatube.XO * nr)()
+    prc_o, xo_o = (orc.HQ * nr)(), (orc.XO * nr)()
+    keys_e, keys_o = np.zeros(nr, np.uint32), np.zeros(nr, np.uint32)
+    for r in range(nr):
+        k = C.c_uint32()
+        eatube.lib().ea_init_generators(1000 + r, C.byref(prc_e[r]),
+                                        C.byref(xo_e[r]), C.byref(k))
+        keys_e[r] = k.value
+        orc.lib().orc_init_generators(C.byref(prc_o[r]), C.byref(xo_o[r]),
+                                      orc.ptr(keys_o[r:], C.c_uint32),
+                                      1000 + r)
+    assert np.array_equal(keys_e, keys_o)
+    assert bytes(prc_e) == bytes(prc_o) and bytes(xo_e) == bytes(xo_o)
+    nhalf, iter0 = 70, 32760
+    out = (eatube.Counter * (nhalf * nr))()
+    eatube.lib().ea_fill_counters(prc_e, xo_e, nr, iter0, nhalf, job, out)
+    got = np.frombuffer(bytes(out), np.uint32).reshape(nhalf, nr, 4)
+    buf = np.zeros(4 * nr, np.uint32)
+    for k in range(nhalf):
+        orc.lib().orc_fill_bits(xo_o, prc_o, nr, iter0 + k, k & 1, job,
+                                orc.ptr(buf, C.c_uint32))
+        assert np.array_equal(got[k], buf.reshape(nr, 4)), k
+
+
+def test_counters_against_reference_fixture():
+    g = Golden("ref_L8_Lz16_NB4_PBC.npz")
+    nr = g.meta["nr"]
+    s = orc.Sim(g.meta["L"], g.meta["Lz"], nr, g.meta["nt"], 1,
+                seeds=tuple(g.meta["seeds"]), job_id=g.meta["job"],
+                betas=g.betas)
+    prc = (eatube.HQ * nr).from_buffer_copy(bytes(s.field("prc")))
+    xo = (eatube.XO * nr).from_buffer_copy(bytes(s.field("xo")))
+    iters = np.frombuffer(g.raw("fill_iters"), np.int32)
+    # the fixture's first 32 entries are consecutive half-sweeps 0..31
+    out = (eatube.Counter * (32 * nr))()
+    eatube.lib().ea_fill_counters(prc, xo, nr, 0, 32, g.meta["job"], out)
+    got = np.frombuffer(bytes(out), np.uint32).reshape(32, nr, 4)
+    assert np.array_equal(iters[:32], np.arange(32))
+    g.check("fill_blocks", _fixture_prefix(g, got))
+
+
+def _fixture_prefix(g, got):
+    """bytes of the fixture's fill_blocks with the first 32 rows replaced by
+    `got`: equal to the fixture iff those rows match"""
+    nr = g.meta["nr"]
+    s = orc.Sim(g.meta["L"], g.meta["Lz"], nr, g.meta["nt"], 1,
+                seeds=tuple(g.meta["seeds"]), job_id=g.meta["job"],
+                betas=g.betas)
+    prc = (orc.HQ * nr).from_buffer_copy(bytes(s.field("prc")))
+    xo = (orc.XO * nr).from_buffer_copy(bytes(s.field("xo")))
+    iters = np.frombuffer(g.raw("fill_iters"), np.int32)
+    rows = []
+    buf = np.zeros(4 * nr, np.uint32)
+    for k, it in enumerate(iters):
+        orc.lib().orc_fill_bits(xo, prc, nr, int(it), k & 1, g.meta["job"],
+                                orc.ptr(buf, C.c_uint32))
+        rows.append(got[k] if k < 32 else buf.reshape(nr, 4).copy())
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("fname", ["ref_L8_Lz16_NB4_PBC.npz",
+                                   "ref_L8_Lz16_NB4_OBC.npz",
+                                   "ref_L8_Lz24_NB3_OBC_WEAK.npz",
+                                   "ref_L16_Lz8_NB3_OBC_STRONG.npz",
+                                   "ref_L8_Lz16_NB4_PBC_MATTIS.npz",
+                                   "ref_L8_Lz16_NB4_OBC_MATTIS.npz",
+                                   "ref_L16_Lz64_NB24_OBC.npz"])
+def test_init_sample_against_reference(fname):
+    g = Golden(fname)
+    m = g.meta
+    seeds_J = eatube.seeds_from_one(m["seeds"][0], 32)
+    seeds_u = eatube.seeds_from_one(m["seeds"][1], 32)
+    g.check("seeds_J", seeds_J[:m["nbits"]])
+    g.check("seeds_u", seeds_u[:m["nbits"]])
+    counts = []
+    for i in range(m["nbits"]):
+        ev, od, J, cp, cnt = eatube.init_sample(
+            m["L"], m["Lz"], m["nr"], m["nt"], m["bc"], m["variant"],
+            m["mattis"], int(seeds_J[i]), int(seeds_u[i]))
+        g.check("even_%d" % i, ev)
+        g.check("odd_%d" % i, od)
+        g.check("Jw_%d" % i, J)
+        g.check("couplings_%d" % i, cp)
+        counts.append(cnt)
+    g.check("countJ", np.array(counts, np.int32))
+
+
+def test_unpack_matches_oracle():
+    L, Lz, nr, nt = 16, 24, 4, 3
+    ev, od, _, _, _ = eatube.init_sample(L, Lz, nr, nt, 1, 0, 0, 5, 6)
+    uu = orc.unpack_spins(L, Lz, nr, nt, ev, od).reshape(nr, nt, -1)
+    for r in range(nr):
+        for c in range(nt):
+            assert np.array_equal(
+                eatube.unpack_lattice(L, Lz, nr, nt, ev, od, r, c), uu[r, c])
+
+
+def test_lut_matches_shipped_file():
+    info = json.load(open(os.path.join(GOLDEN, "lut_nb40.json")))
+    betas = np.array(info["betas"], np.float64)
+    blob = betas.tobytes() + eatube.make_luts(betas).tobytes()
+    assert hashlib.sha256(blob).hexdigest() == info["sha256"]
+
+
+@pytest.mark.parametrize("nr", [4, 8, 16])
+def test_houdayer_pairs_match_oracle(nr):
+    betas = orc.betas_linear(0.6, 1.6, 4)
+    s = orc.Sim(8, 8, nr, 4, 1, seeds=(11, 12, 13, 99), betas=betas)
+    cl = eatube.hq_init(99)
+    for _ in range(5):
+        s.houdayer()
+        want = s.field("last_pairs")
+        got = np.zeros(4, np.int32)
+        eatube.check(eatube.lib().ea_houdayer_pairs(
+            C.byref(cl), nr, got.ctypes.data_as(C.POINTER(C.c_int32))))
+        assert np.array_equal(got, want)
+        assert got[0] != got[1] and got[2] != got[3]
+        assert max(got[:2]) < nr // 2 <= min(got[2:])
+
+
+def test_pt_seeds_are_sign_extended_int():
+    """Init_Random stores the PT seed in an int (ini.c:63,90): the product
+    Simulation must derive the same PT generators as the reference."""
+    g = Golden("ref_L8_Lz16_NB4_PBC.npz")
+    m = g.meta
+    nr, ns = m["nr"], m["nbits"]
+    seeds_MC = eatube.seeds_from_one(m["seeds"][2], nr)
+    g.check("seeds_MC", seeds_MC)
+    pts = (eatube.HQ * (ns * nr))()
+    for r in range(nr):
+        t = eatube.seeds_from_one(int(seeds_MC[r]), 32)
+        for i in range(ns):
+            s32 = int(t[i]) & 0xFFFFFFFF
+            if s32 >= 2**31:
+                s32 -= 2**32
+            eatube.lib().ea_hq_init(C.byref(pts[i * nr + r]),
+                                    s32 & (2**64 - 1))
+    g.check("pt_init", bytes(pts))
