@@ -116,8 +116,8 @@ def cpu_baseline(w, S, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="obc_L16_Lz64",
                     choices=sorted(WORKLOADS))
     ap.add_argument("--samples", type=int, default=31)

@@ -218,12 +218,23 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   if (!t.val.empty())
     HIP_TRY(hipMemcpy(e->tab, t.val.data(), sizeof(double) * t.val.size(), hipMemcpyHostToDevice));
 
-  // Houdayer: tubes beyond one workgroup's LDS are labelled slab by slab
-  if (g.ctm > 0 && g.V > kHouWholeMax) {
-    e->planes = std::max(1, kHouWholeMax / (g.L * g.L));
-    const size_t inst = (size_t)2 * g.ns * g.ctm;
-    if ((rc = dalloc(&e->hlab, inst * g.V)) || (rc = dalloc(&e->hcb, inst * (g.V / 32))))
-      return rc;
+  // Houdayer: a tube whose labels fit one workgroup's LDS is labelled there,
+  // larger tubes slab by slab with a merge in HBM
+  if (g.ctm > 0) {
+    if (g.L > kHouMaxL)
+      return fail(EA_E_ARG, "the Houdayer move supports L <= %d (L=%d)", kHouMaxL, g.L);
+    int lds_block = 0, lds_optin = 0;
+    HIP_TRY(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    HIP_TRY(hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, dev));
+    const int lds_max = std::max(lds_block, lds_optin);
+    if (hou_whole_lds(g) > (size_t)lds_max) {
+      int planes = std::max(1, std::min(g.Lz, kHouSlabSites / (g.L * g.L)));
+      while (planes > 1 && hou_slab_lds(g, planes) > (size_t)lds_max) --planes;
+      e->planes = planes;
+      const size_t inst = (size_t)2 * g.ns * g.ctm;
+      if ((rc = dalloc(&e->hlab, inst * g.V)) || (rc = dalloc(&e->hcb, inst * (g.V / 32))))
+        return rc;
+    }
   }
   *out = e.release();
   return EA_OK;
@@ -401,6 +412,8 @@ static int enqueue_houdayer(ea_engine *e, const int32_t pairs[4], uint64_t seed,
     return fail(EA_E_ARG, "a Houdayer pair needs two distinct replicas");
   HouArgs a{};
   a.spin = e->spin;
+  a.J = e->J;
+  a.ener = e->ener;
   a.wctb = e->wctb;
   std::memcpy(a.pairs, pairs, sizeof a.pairs);
   a.seed = seed;
@@ -507,8 +520,10 @@ extern "C" int ea_run(ea_engine *e, const ea_schedule *sch, uint64_t *rng_offset
         offset += per_call;
       }
     }
-    // after a cluster move the energies changed; otherwise the sweep's are current
-    if (sch->cluster) {
+    // the whole-tube cluster move updates the energies it changes; the slab
+    // path leaves them to a full energy pass; without cluster moves the
+    // sweep's energies are current (PT only permutes clones)
+    if (sch->cluster && e->planes > 0) {
       Timed t(e, EA_K_ENERGY);
       HIP_TRY(launch_energy(g, e->spin, e->J, e->ener, e->stream));
     }

@@ -8,6 +8,9 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <utility>
+#include <vector>
 
 namespace eatube {
 
@@ -152,9 +155,9 @@ __device__ __forceinline__ Nbrs neighbours(const Lattice &g, int p, int h) {
 __device__ __forceinline__ void load_neighbours(const Lattice &g,
                                                 const uint64_t *old, int r,
                                                 const Nbrs &nb, int parity,
-                                                uint64_t sc[6]) {
+                                                uint64_t sc[6], int stride) {
 #pragma unroll
-  for (int d = 0; d < 6; ++d) sc[d] = old[(size_t)nb.h[d] * g.nr + r];
+  for (int d = 0; d < 6; ++d) sc[d] = old[(size_t)nb.h[d] * stride + r];
   if (nb.wrap) {
     if (nb.wrap & 1) sc[0] = wrap_xp(sc[0]);
     if (nb.wrap & 2) sc[1] = wrap_yp(sc[1]);
@@ -179,50 +182,72 @@ __device__ __forceinline__ int replica_sum(int e, int nr) {
 
 // ------------------------------------------------------------------ sweep
 
-// One half-sweep of every sample.  Thread (beta, h, r) draws the three
+// One half-sweep of every sample.  A block owns 64 consecutive (h, r)
+// positions of one beta.  First 384 of its threads each make one of the six
 // random words of the reference thread tid = beta*n*nr + h*nr + r
-// (d_gen_rndbits_cuda, main.cu:1095-1150) and applies them to the clone that
-// sits at beta in each sample (d_oneMCstepBN_multisample,
-// main.cu:1385-1510): the words are shared by all samples, so they are made
-// once and never leave registers.
-__global__ void __launch_bounds__(256) k_sweep(Lattice g, SweepArgs a) {
+// (d_gen_rndbits_cuda, main.cu:1095-1150) with the beta's LUT in LDS; then
+// its 16 waves update 16 samples at a time, each wave the clone that sits at
+// beta in its sample (d_oneMCstepBN_multisample, main.cu:1385-1510).  The
+// random words are shared by all samples, are made once and never reach HBM.
+constexpr int kSwPos = 64;                  // positions per block (one wave)
+constexpr int kSwSlots = 16;                // samples in flight per position
+constexpr int kSwThreads = kSwPos * kSwSlots;
+
+__global__ void __launch_bounds__(kSwThreads) k_sweep(Lattice g, SweepArgs a) {
   __shared__ uint32_t lut[EA_LUT_BYTES / 4];
   __shared__ uint8_t clone_at[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
-  const int beta = blockIdx.y;
+  __shared__ uint32_t words[kSwPos][6];
+  __shared__ uint64_t ids[kSwPos][2];
+  __shared__ uint32_t ctr[EA_MAX_REPLICAS][5];
+  const int beta = blockIdx.y, tx = threadIdx.x;
   const uint32_t *src =
       reinterpret_cast<const uint32_t *>(a.lut + (size_t)beta * EA_LUT_BYTES);
-  for (int i = threadIdx.x; i < EA_LUT_BYTES / 4; i += blockDim.x) lut[i] = src[i];
-  for (int i = threadIdx.x; i < g.ns * g.nr; i += blockDim.x)
+  for (int i = tx; i < EA_LUT_BYTES / 4; i += kSwThreads) lut[i] = src[i];
+  for (int i = tx; i < g.ns * g.nr; i += kSwThreads)
     clone_at[i] = a.wctb[(size_t)i * g.nt + beta];
+  if (tx < g.nr) {
+    ctr[tx][0] = a.ctr[tx].x;
+    ctr[tx][1] = a.ctr[tx].y;
+    ctr[tx][2] = a.ctr[tx].z;
+    ctr[tx][3] = a.ctr[tx].w;
+    ctr[tx][4] = a.keys[tx];
+  }
   if (a.energy_mode == 1 && blockIdx.x == 0 && blockIdx.y == 0)
-    for (int i = threadIdx.x; i < g.ns * g.nr * g.nt; i += blockDim.x) a.ener[i] = 0;
+    for (int i = tx; i < g.ns * g.nr * g.nt; i += kSwThreads) a.ener[i] = 0;
   __syncthreads();
 
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nwords = g.n * g.nr;
-  const bool valid = t < nwords;
-  const int h = valid ? t >> g.nr_shift : 0, r = t & (g.nr - 1);
-
-  uint64_t id1 = 0, id2 = 0;
-  if (valid) {
-    const uint32_t tid = (uint32_t)beta * (uint32_t)nwords + (uint32_t)t;
-    const ea_counter c = a.ctr[r];
-    const uint32_t key = a.keys[r], z = c.z + tid;
-    uint32_t w[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] = rnd_word(c.x + k, c.y, z, c.w, tid, key, lut);
+  const int nwords = g.n * g.nr, p0 = blockIdx.x * kSwPos;
+  if (tx < 6 * kSwPos) {
+    const int pos = tx / 6, k = tx - 6 * pos, t = p0 + pos;
+    if (t < nwords) {
+      const int r = t & (g.nr - 1);
+      const uint32_t tid = (uint32_t)beta * (uint32_t)nwords + (uint32_t)t;
+      words[pos][k] = rnd_word(ctr[r][0] + k, ctr[r][1], ctr[r][2] + tid,
+                               ctr[r][3], tid, ctr[r][4], lut);
+    }
+  }
+  __syncthreads();
+  if (tx < kSwPos) {
+    const uint32_t *w = words[tx];
     const uint64_t b0 = ((uint64_t)w[0] << 32) | w[1];
     const uint64_t b1 = (((uint64_t)w[2] << 32) | w[3]) & b0;
     const uint64_t b2 = (((uint64_t)w[4] << 32) | w[5]) & b1;
     // id = id1 + 2 id2 in 0..3 with P(id >= k) = exp(-4 k beta)
-    id2 = b1;
-    id1 = (b0 ^ b1) | b2;
+    ids[tx][0] = (b0 ^ b1) | b2;
+    ids[tx][1] = b1;
   }
+  __syncthreads();
+
+  const int pos = tx & (kSwPos - 1), slot = tx / kSwPos;
+  const int t = p0 + pos;
+  const bool valid = t < nwords;
+  const int h = valid ? t >> g.nr_shift : 0, r = t & (g.nr - 1);
+  const uint64_t id1 = ids[pos][0], id2 = ids[pos][1];
   const int p = a.parity;
   const Nbrs nb = neighbours(g, p, h);
-  const bool wave_live = (int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)) < nwords;
+  const bool wave_live = p0 < nwords;
 
-  for (int s = 0; s < g.ns; ++s) {
+  for (int s = slot; s < g.ns; s += kSwSlots) {
     const int clone = clone_at[s * g.nr + r];
     int e = 0;
     if (valid) {
@@ -231,7 +256,7 @@ __global__ void __launch_bounds__(256) k_sweep(Lattice g, SweepArgs a) {
       const uint64_t *old = base + (size_t)(1 - p) * g.half;
       const uint64_t sv = cur[(size_t)h * g.nr + r];
       uint64_t sc[6];
-      load_neighbours(g, old, r, nb, p, sc);
+      load_neighbours(g, old, r, nb, p, sc, g.nr);
       const uint64_t *Jw = a.J + ((size_t)(2 * s + p) * g.n + h) * 6;
       uint64_t f[6];
 #pragma unroll
@@ -256,7 +281,7 @@ __global__ void __launch_bounds__(256) k_sweep(Lattice g, SweepArgs a) {
     if (a.energy_mode == 2) {
       // every bond has exactly one odd end: the odd half-sweep sees them all
       e = replica_sum(e, g.nr);
-      if (wave_live && (threadIdx.x & 63) < g.nr)
+      if (wave_live && pos < g.nr)
         atomicAdd(a.ener + ((size_t)s * g.nr + r) * g.nt + clone, e);
     }
   }
@@ -279,7 +304,7 @@ __global__ void __launch_bounds__(256) k_energy(Lattice g, const uint64_t *spin,
     const uint64_t sv = base[(size_t)h * g.nr + r];
     const Nbrs nb = neighbours(g, 0, h);
     uint64_t sc[6];
-    load_neighbours(g, base + g.half, r, nb, 0, sc);
+    load_neighbours(g, base + g.half, r, nb, 0, sc, g.nr);
     const uint64_t *Jw = J + ((size_t)(2 * s) * g.n + h) * 6;
     int u = 0;
 #pragma unroll
@@ -304,47 +329,78 @@ __device__ __forceinline__ double exact_exp(const PtArgs &a, int i, int d) {
 }
 
 // Parallel_Tempering (upd.c:3-48), one wave per (sample, replica) chain.
-// The chain's Parisi-Rapuano state is staged in LDS; lane 0 walks the beta
-// pairs in order.  The swap decision compares FNORM*final with exp(pt): the
-// device exp decides unless the two are within 2^-40 relative, where the
-// host-libm table decides, so every decision equals the reference's.  The
-// statistics are then added by one lane per pair with the tabulated values.
+// The chain's Parisi-Rapuano state is staged in LDS.  Its draws do not
+// depend on the swap decisions, so the wave makes them first: with lags 24,
+// 55 and 61, draws k .. k+23 only read earlier values, so 24 lanes draw at a
+// time.  Then each pair's uniform u = FNORM*final and log(u) are formed in
+// parallel, and lane 0 walks the pairs in order: swap when pt >= 0, else when
+// log(u) < pt -- unless the two are within 1e-12 (or u = 0), where the exact
+// comparison u < exp(pt) is made with the host libm's exp from the table.
+// Both sides' errors are far below 1e-12, so every decision equals the
+// reference's.  The statistics are then added by one lane per pair with the
+// tabulated exp values.
 __global__ void __launch_bounds__(64) k_pt(Lattice g, PtArgs a) {
   __shared__ uint64_t ira[256];
+  __shared__ uint64_t zs[EA_MAX_BETAS], prs[EA_MAX_BETAS];
+  __shared__ double uu[EA_MAX_BETAS], lu[EA_MAX_BETAS];
   __shared__ int32_t E[EA_MAX_BETAS];
   __shared__ uint8_t perm[EA_MAX_BETAS + 1];
   __shared__ int32_t dE[EA_MAX_BETAS];
   __shared__ uint8_t took[EA_MAX_BETAS];
-  const int chain = blockIdx.x, nt = g.nt;
+  const int chain = blockIdx.x, nt = g.nt, np = nt - 1, lane = threadIdx.x;
   ea_hq_state *st = a.st + chain;
-  for (int i = threadIdx.x; i < 256; i += 64) ira[i] = st->ira[i];
-  for (int i = threadIdx.x; i < nt; i += 64) {
+  for (int i = lane; i < 256; i += 64) ira[i] = st->ira[i];
+  for (int i = lane; i < nt; i += 64) {
     E[i] = a.ener[(size_t)chain * nt + i];
     perm[i] = a.wctb[(size_t)chain * nt + i];
   }
+  const uint8_t ip = st->ip;
+  if (lane == 0) {  // the congruential half is a cheap serial recurrence
+    uint64_t z = st->zseed;
+    for (int k = 0; k < np; ++k) zs[k] = z = z * kLcg + 1;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint8_t ip = st->ip, ip1 = st->ip1, ip2 = st->ip2, ip3 = st->ip3;
-    uint64_t zseed = st->zseed, PR = st->PR, fin = st->final_;
-    for (int i = 0; i + 1 < nt; ++i) {
-      ira[ip] = ira[ip1] + ira[ip2];
-      PR = ira[ip] ^ ira[ip3];
-      zseed = zseed * kLcg + 1;
-      fin = zseed + PR;
-      ++ip;
-      ++ip1;
-      ++ip2;
-      ++ip3;
+  // a ring slot is rewritten 256 draws after it was written, and every
+  // value a draw reads was made 24 to 61 draws before it: each round reads
+  // only slots of earlier rounds that no later draw has touched yet
+  for (int k0 = 0; k0 < np; k0 += 24) {
+    const int k = k0 + lane;
+    const bool act = lane < 24 && k < np;
+    uint64_t nv = 0, pr = 0;
+    if (act) {
+      nv = ira[(uint8_t)(ip + k - 24)] + ira[(uint8_t)(ip + k - 55)];
+      pr = nv ^ ira[(uint8_t)(ip + k - 61)];
+    }
+    __syncthreads();
+    if (act) {
+      ira[(uint8_t)(ip + k)] = nv;
+      prs[k] = pr;
+    }
+    __syncthreads();
+  }
+  for (int k = lane; k < np; k += 64) {
+    const uint64_t fin = zs[k] + prs[k];
+    const double u = kFnorm * (double)fin;
+    uu[k] = u;
+    lu[k] = log(u);
+    if (k == np - 1) {
+      st->final_ = fin;
+      st->PR = prs[k];
+      st->zseed = zs[k];
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    for (int i = 0; i < np; ++i) {
       const int d = E[perm[i + 1]] - E[perm[i]];
       const double pt = a.nd[i] * (double)d;
       bool take = true;
       if (!(pt >= 0)) {
-        const double u = kFnorm * (double)fin;
-        const double ex = exp(pt);
-        if (u == 0.0 || fabs(u - ex) <= ex * 0x1p-40)
+        const double u = uu[i], l = lu[i];
+        if (u == 0.0 || fabs(l - pt) <= 1e-12)
           take = u < exact_exp(a, i, d);
         else
-          take = u < ex;
+          take = l < pt;
       }
       dE[i] = d;
       took[i] = take;
@@ -354,19 +410,16 @@ __global__ void __launch_bounds__(64) k_pt(Lattice g, PtArgs a) {
         perm[i + 1] = tmp;
       }
     }
-    st->ip = ip;
-    st->ip1 = ip1;
-    st->ip2 = ip2;
-    st->ip3 = ip3;
-    st->zseed = zseed;
-    st->PR = PR;
-    st->final_ = fin;
+    st->ip = (uint8_t)(ip + np);
+    st->ip1 = (uint8_t)(st->ip1 + np);
+    st->ip2 = (uint8_t)(st->ip2 + np);
+    st->ip3 = (uint8_t)(st->ip3 + np);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 256; i += 64) st->ira[i] = ira[i];
-  for (int i = threadIdx.x; i < nt; i += 64) a.wctb[(size_t)chain * nt + i] = perm[i];
-  for (int i = threadIdx.x; i + 1 < nt; i += 64) {
-    const size_t k = (size_t)chain * (nt - 1) + i;
+  for (int i = lane; i < 256; i += 64) st->ira[i] = ira[i];
+  for (int i = lane; i < nt; i += 64) a.wctb[(size_t)chain * nt + i] = perm[i];
+  for (int i = lane; i < np; i += 64) {
+    const size_t k = (size_t)chain * np + i;
     const double pt = a.nd[i] * (double)dE[i];
     a.att[k] += 1.0;
     a.acc[k] += pt >= 0 ? 1.0 : exact_exp(a, i, dE[i]);
@@ -401,14 +454,23 @@ __global__ void k_fill_f64(double *p, double v, size_t n) {
 
 // --------------------------------------------------------------- Houdayer
 
-// Cluster coin: U < 1/2 with U from Philox4x32-10 at counter
-// (idx_lo, idx_hi, 0x486f7564, 0) and key (seed_lo, seed_hi).  The index
-// scheme follows the reference's cuRAND use (HoudayerXEA.cu:174-195,373):
-// idx = offset + s*V*NT + beta*V + site.
-__device__ __forceinline__ bool coin(uint64_t seed, uint64_t idx) {
-  uint32_t v0 = (uint32_t)idx, v1 = (uint32_t)(idx >> 32), v2 = 0x486f7564u, v3 = 0;
+// Cluster coins.  The reference draws cuRAND XORWOW doubles
+// (HoudayerXEA.cu:174-195,373): coin[site] = U(idx) < 1/2 with
+// idx = offset + s*V*NT + beta*V + site.  Here idx keeps that scheme and the
+// coin is bit (idx mod 128) of Philox4x32-10 at counter
+// (b_lo, b_hi, 0x486f7564, 0), b = idx / 128, key (seed_lo, seed_hi): one
+// Philox call serves 128 consecutive sites.  coin_word returns the coins of
+// the 32 indices idx0 .. idx0+31 (idx0 a multiple of 32).
+__device__ __forceinline__ uint32_t coin_word(uint64_t seed, uint64_t idx0) {
+  const uint64_t b = idx0 >> 7;
+  uint32_t v0 = (uint32_t)b, v1 = (uint32_t)(b >> 32), v2 = 0x486f7564u, v3 = 0;
   philox10(v0, v1, v2, v3, (uint32_t)seed, (uint32_t)(seed >> 32));
-  return v0 < 0x80000000u;
+  switch ((idx0 >> 5) & 3) {
+    case 0: return v0;
+    case 1: return v1;
+    case 2: return v2;
+    default: return v3;
+  }
 }
 
 // Word offsets of replicas r1, r2 of the instance (group, sample, beta).
@@ -448,11 +510,29 @@ __device__ __forceinline__ uint32_t find_root(const uint32_t *lab, uint32_t a) {
   return a;
 }
 
+// Root of a with path splitting: every node passed is re-pointed at its
+// grandparent.  The re-pointing is an atomicMin to an ancestor, so it never
+// cuts a node from its tree, and a link it displaces was made by a union
+// that does not rely on it (see unite).
+__device__ __forceinline__ uint32_t find_split(uint32_t *lab, uint32_t a) {
+  const volatile uint32_t *v = lab;
+  uint32_t p = v[a];
+  while (p != a) {
+    const uint32_t gp = v[p];
+    if (gp != p) atomicMin(lab + a, gp);
+    a = p;
+    p = gp;
+  }
+  return a;
+}
+
 // Link the trees of a and b, the larger root under the smaller one; a link
 // displaced by a concurrent union is re-linked (Komura, CPC 194 (2015) 54).
+// A union only relies on its own link when it found a root there; nodes
+// that are not roots are only ever re-pointed at ancestors.
 __device__ __forceinline__ void unite(uint32_t *lab, uint32_t a, uint32_t b) {
-  a = find_root(lab, a);
-  b = find_root(lab, b);
+  a = find_split(lab, a);
+  b = find_split(lab, b);
   while (a != b) {
     if (a < b) {
       const uint32_t t = a;
@@ -469,91 +549,197 @@ __device__ __forceinline__ uint32_t qbit(const uint32_t *qb, int i) {
   return (qb[i >> 5] >> (i & 31)) & 1u;
 }
 
-// Union along +x, +y (periodic) and +z of local site i of a slab of planes
-// [z0, z1) whose labels start at linear site z0*L*L.  z bonds leaving the
-// slab are skipped unless the slab is the whole periodic tube.
-__device__ __forceinline__ void slab_unions(const Lattice &g, uint32_t *lab,
-                                            const uint32_t *qb, int i, int planes,
-                                            bool wrap_z) {
-  const int L = g.L, S = L * L;
-  const int x = i % L, yz = i / L, y = yz % L, z = yz / L;
-  const uint32_t qi = qbit(qb, i);
-  int j = x + 1 == L ? i - (L - 1) : i + 1;
-  if (qbit(qb, j) == qi) unite(lab, i, j);
-  j = y + 1 == L ? i - L * (L - 1) : i + L;
-  if (qbit(qb, j) == qi) unite(lab, i, j);
-  if (z + 1 < planes) {
-    j = i + S;
-    if (qbit(qb, j) == qi) unite(lab, i, j);
-  } else if (wrap_z) {
-    j = i - S * (planes - 1);
-    if (qbit(qb, j) == qi) unite(lab, i, j);
+// Cluster labelling of a slab of `planes` full z-planes in LDS, all threads
+// of the block taking part.  rows[r] holds the q bits of row r = y + L*z
+// (bit x; L <= 64); lab[i] is the label of local site i = x + L*r.  Label
+// values are local site indices and end as the smallest index of each
+// equal-q cluster:
+//  1. runs of equal q along the periodic x row point at their first site
+//     (a run wrapping past x = L-1 points at the row start); no atomics;
+//  2. y and z bonds between rows, from bit masks: a bond at x is united
+//     only where a run overlap starts (when the bond at x-1 joins the same
+//     two runs the union is implied);
+//  3. flatten.
+// z bonds leave the slab only when wrap_z (the whole periodic tube).
+__device__ void label_rows(const Lattice &g, uint32_t *lab, const uint64_t *rows,
+                           int planes, bool wrap_z) {
+  const int L = g.L, nrows = planes * L, T = blockDim.x, tid = threadIdx.x;
+  const uint64_t full = L == 64 ? ~0ULL : ((1ULL << L) - 1);
+  for (int i = tid; i < nrows * L; i += T) {
+    const int r = i / L, x = i - r * L, base = i - x;
+    const uint64_t q = rows[r];
+    const uint64_t bnd = ((q ^ (q << 1)) & full) | 1ULL;  // run starts
+    int start = 63 - __clzll(bnd & ((2ULL << x) - 1));
+    // a last run continuing through the periodic wrap joins the first
+    const int last = 63 - __clzll(bnd);
+    if (last != 0 && start == last && !(((q >> (L - 1)) ^ q) & 1)) start = 0;
+    lab[i] = base + start;
   }
+  __syncthreads();
+  for (int r = tid; r < nrows; r += T) {
+    const uint64_t q = rows[r];
+    const uint64_t lq = ~(q ^ (q << 1)) & full & ~1ULL;  // q[x] == q[x-1]
+    const int y = r % L, zl = r / L;
+    auto link = [&](int r2) {
+      const uint64_t q2 = rows[r2];
+      const uint64_t l2 = ~(q2 ^ (q2 << 1)) & full & ~1ULL;
+      uint64_t todo = ~(q ^ q2) & full & ~(lq & l2);
+      while (todo) {
+        const int x = __ffsll((unsigned long long)todo) - 1;
+        todo &= todo - 1;
+        unite(lab, r * L + x, r2 * L + x);
+      }
+    };
+    link(y + 1 == L ? r - (L - 1) : r + 1);
+    if (zl + 1 < planes)
+      link(r + L);
+    else if (wrap_z)
+      link(r - L * (planes - 1));
+  }
+  __syncthreads();
+  for (int i = tid; i < nrows * L; i += T) lab[i] = find_root(lab, i);
+  __syncthreads();
+}
+
+// q bits of row (y, z) from the two replicas' words: the L sites of a row
+// use ML words (parity alternating with x), each for its 4 x copies (bx).
+template <typename Load>
+__device__ __forceinline__ uint64_t row_bits(const Lattice &g, int y, int z,
+                                             Load load) {
+  const int by = y / g.ML, yb = y - by * g.ML, bz = z / g.MLz, zg = z - bz * g.MLz;
+  const int hrow = g.ML * yb + g.MS * zg, bitrow = 4 * by + 16 * bz;
+  const int prow = (y ^ z) & 1;
+  uint64_t bits = 0;
+  for (int ax = 0; ax < g.ML; ++ax) {
+    const uint64_t q = load((ax + prow) & 1, (ax + hrow) >> 1) >> bitrow;
+#pragma unroll
+    for (int bx = 0; bx < 4; ++bx)
+      bits |= ((q >> bx) & 1ULL) << (bx * g.ML + ax);
+  }
+  return bits;
+}
+
+// Move bits of word (p, h) gathered from the linear move bitmap mb (bit i
+// of site i); 4 threads per word, thread `quarter` gathering the 16 copies
+// of z block bz = quarter.
+__device__ __forceinline__ uint64_t word_move_bits(const Lattice &g,
+                                                   const uint64_t *mb, int p,
+                                                   int h, int quarter) {
+  const int gz = h / g.hs, rem = h - gz * g.hs, yb = rem / g.hl;
+  const int ax = 2 * (rem - yb * g.hl) + ((p + yb + gz) & 1);
+  const int zoff = g.L * g.L * (quarter * g.MLz + gz);
+  uint64_t m = 0;
+#pragma unroll
+  for (int by = 0; by < 4; ++by) {
+    const int rowbase = zoff + g.L * (by * g.ML + yb) + ax;
+#pragma unroll
+    for (int bx = 0; bx < 4; ++bx) {
+      const int site = rowbase + bx * g.ML;
+      m |= ((mb[site >> 6] >> (site & 63)) & 1ULL) << (bx + 4 * by + 16 * quarter);
+    }
+  }
+  return m;
 }
 
 // Whole-tube Houdayer move of one instance (group, sample, beta < ctm) in a
 // single workgroup: overlap q = s1 ^ s2 of replicas r1, r2 at beta
-// (d_computeoverlap), clusters of equal q labelled by union-find in LDS
+// (d_computeoverlap), clusters of equal q labelled in LDS
 // (init_YK_custom/ReduceLabels/analysis, HoudayerXEA.cu:77-121,233-354),
 // label = minimum linear site index, one coin per cluster root
 // (spin_select, :358-374), then q' = q ^ coin(label) and s1 ^= q', s2 ^= q'
 // (spin_flip_YK_custom + d_pack + d_useoverlap).
-template <int MAXV>
+// The kernel also recomputes the energies of the two configurations it
+// changed (the Measure_Energy that follows a cluster move, main.cu:209), so
+// no separate energy pass is needed.
+// Dynamic LDS: rows[V/L] u64 | qw[2n] u64 | nw[2n] u64 | cb[V/32] u32 (coin
+// of every site) | lab[V] u32.
 __global__ void __launch_bounds__(1024) k_hou_whole(Lattice g, HouArgs a) {
-  __shared__ uint32_t lab[MAXV];
-  __shared__ uint32_t qb[MAXV / 32];
-  __shared__ uint32_t cb[MAXV / 32];
-  __shared__ int offs[64];
+  extern __shared__ uint64_t lds[];
+  const int V = g.V, nrows = V / g.L, T = blockDim.x, tid = threadIdx.x;
+  uint64_t *rows = lds;
+  uint64_t *qw = rows + nrows;  // q words, then replica r1's new words
+  uint64_t *nw = qw + 2 * g.n;  // replica r2's new words
+  uint32_t *cb = reinterpret_cast<uint32_t *>(nw + 2 * g.n);
+  uint32_t *lab = cb + V / 32;
+  __shared__ int esum[2];
   const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
-  const int V = g.V, T = blockDim.x, tid = threadIdx.x;
   const Pair pw = pair_words(g, a, s, grp, beta);
-  for (int i = tid; i < V / 32; i += T) {
-    qb[i] = 0;
-    cb[i] = 0;
-  }
-  if (tid < 64) offs[tid] = copy_offset(g, tid);
-  for (int i = tid; i < V; i += T) lab[i] = i;
-  __syncthreads();
+  if (tid < 2) esum[tid] = 0;
   for (int w = tid; w < 2 * g.n; w += T) {
     const int p = w >= g.n, h = w - p * g.n;
     const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
-    uint64_t q = a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
-    const int base = site_base(g, p, h);
-    while (q) {
-      const int b = __ffsll((unsigned long long)q) - 1;
-      q &= q - 1;
-      const int site = base + offs[b];
-      atomicOr(qb + (site >> 5), 1u << (site & 31));
-    }
+    qw[w] = a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
   }
   __syncthreads();
-  for (int i = tid; i < V; i += T) slab_unions(g, lab, qb, i, g.Lz, g.bc == EA_PBC);
+  for (int r = tid; r < nrows; r += T)
+    rows[r] = row_bits(g, r % g.L, r / g.L,
+                       [&](int p, int h) { return qw[p * g.n + h]; });
   __syncthreads();
-  for (int i = tid; i < V; i += T) lab[i] = find_root(lab, i);
-  __syncthreads();
+  label_rows(g, lab, rows, g.Lz, g.bc == EA_PBC);
   const uint64_t cidx = a.offset0 + (uint64_t)grp * V * g.nt * g.ns +
                         (uint64_t)s * V * g.nt + (uint64_t)beta * V;
-  for (int i = tid; i < V; i += T)
-    if (lab[i] == (uint32_t)i && coin(a.seed, cidx + i))
-      atomicOr(cb + (i >> 5), 1u << (i & 31));
+  for (int c = tid; c < V / 32; c += T) cb[c] = coin_word(a.seed, cidx + 32 * c);
   __syncthreads();
   if (a.labels_out) {
     uint32_t *out = a.labels_out + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
     for (int i = tid; i < V; i += T) out[i] = lab[i];
   }
-  for (int w = tid; w < 2 * g.n; w += T) {
+  for (int i = tid; i < V; i += T) {  // rows now receive the move bits,
+    const uint32_t root = lab[i];     // as a linear bitmap (64 sites/word)
+    const uint64_t m = __ballot((cb[root >> 5] >> (root & 31)) & 1u);
+    if ((tid & 63) == 0) rows[i >> 6] = m;
+  }
+  __syncthreads();
+  for (int t0 = 0; t0 < 8 * g.n; t0 += T) {  // every lane reaches the shuffles
+    const int t = t0 + tid;
+    const bool live = t < 8 * g.n;
+    const int w = live ? t >> 2 : 0, quarter = t & 3;
     const int p = w >= g.n, h = w - p * g.n;
-    const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
-    const int base = site_base(g, p, h);
-    uint64_t m = 0;
-    for (int b = 0; b < 64; ++b) {
-      const uint32_t root = lab[base + offs[b]];
-      m |= (uint64_t)((cb[root >> 5] >> (root & 31)) & 1u) << b;
+    uint64_t m = live ? word_move_bits(g, rows, p, h, quarter) : 0;
+    m |= __shfl_xor(m, 1, 64);
+    m |= __shfl_xor(m, 2, 64);
+    if (live && quarter == 0) {
+      const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+      const uint64_t qn = qw[w] ^ m;  // q' = q ^ coin(label)
+      const uint64_t x1 = a.spin[pw.o1 + o] ^ qn, x2 = a.spin[pw.o2 + o] ^ qn;
+      a.spin[pw.o1 + o] = x1;
+      a.spin[pw.o2 + o] = x2;
+      qw[w] = x1;
+      nw[w] = x2;
     }
-    const uint64_t x1 = a.spin[pw.o1 + o], x2 = a.spin[pw.o2 + o];
-    const uint64_t qn = x1 ^ x2 ^ m;
-    a.spin[pw.o1 + o] = x1 ^ qn;
-    a.spin[pw.o2 + o] = x2 ^ qn;
+  }
+  __syncthreads();
+  // energies of the two new configurations, from their even words
+  for (int t0 = 0; t0 < 2 * g.n; t0 += T) {  // every lane reaches the shuffles
+    const int t = t0 + tid;
+    int e0 = 0, e1 = 0;
+    if (t < 2 * g.n) {
+      const int rep = t >= g.n, h = t - rep * g.n;
+      const uint64_t *wd = rep ? nw : qw;
+      const Nbrs nb = neighbours(g, 0, h);
+      uint64_t sc[6];
+      load_neighbours(g, wd + g.n, 0, nb, 0, sc, 1);
+      const uint64_t sv = wd[h];
+      const uint64_t *Jw = a.J + ((size_t)(2 * s) * g.n + h) * 6;
+      int u = 0;
+#pragma unroll
+      for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
+      (rep ? e1 : e0) = 6 * 64 - 2 * u;
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+      e0 += __shfl_xor(e0, off, 64);
+      e1 += __shfl_xor(e1, off, 64);
+    }
+    if ((tid & 63) == 0) {
+      if (e0) atomicAdd(esum, e0);
+      if (e1) atomicAdd(esum + 1, e1);
+    }
+  }
+  __syncthreads();
+  if (tid < 2) {
+    const int r = a.pairs[2 * grp + tid];
+    const int c = a.wctb[((size_t)s * g.nr + r) * g.nt + beta];
+    a.ener[((size_t)s * g.nr + r) * g.nt + c] = esum[tid];
   }
 }
 
@@ -576,29 +762,31 @@ __device__ __forceinline__ size_t instance(const Lattice &g, int grp, int s,
   return ((size_t)grp * g.ns + s) * g.ctm + beta;
 }
 
-template <int MAXV>
+// Dynamic LDS: rows[planes*L] u64 | lab[planes*L*L] u32.
 __global__ void __launch_bounds__(1024) k_hou_slab(Lattice g, HouArgs a) {
-  __shared__ uint32_t lab[MAXV];
-  __shared__ uint32_t qb[MAXV / 32];
+  extern __shared__ uint64_t lds[];
   const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
   const int S = g.L * g.L, z0 = blockIdx.x * a.planes;
   const int planes = min(a.planes, g.Lz - z0), nsite = planes * S;
   const int T = blockDim.x, tid = threadIdx.x;
+  uint64_t *rows = lds;
+  uint32_t *lab = reinterpret_cast<uint32_t *>(rows + a.planes * g.L);
   const Pair pw = pair_words(g, a, s, grp, beta);
   const size_t ib = instance(g, grp, s, beta);
   const int gbase = z0 * S;
-  for (int c = tid; c < nsite / 32; c += T) {
-    uint32_t word = 0;
-    for (int k = 0; k < 32; ++k) word |= q_site(g, a, pw, gbase + 32 * c + k) << k;
-    qb[c] = word;
-    a.cbits[ib * (g.V / 32) + gbase / 32 + c] = 0;
-  }
-  for (int i = tid; i < nsite; i += T) lab[i] = i;
+  for (int r = tid; r < planes * g.L; r += T)
+    rows[r] = row_bits(g, r % g.L, z0 + r / g.L, [&](int p, int h) {
+      const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+      return a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
+    });
+  const uint64_t cidx = a.offset0 + (uint64_t)grp * g.V * g.nt * g.ns +
+                        (uint64_t)s * g.V * g.nt + (uint64_t)beta * g.V + gbase;
+  for (int c = tid; c < nsite / 32; c += T)
+    a.cbits[ib * (g.V / 32) + gbase / 32 + c] = coin_word(a.seed, cidx + 32 * c);
   __syncthreads();
-  for (int i = tid; i < nsite; i += T) slab_unions(g, lab, qb, i, planes, false);
-  __syncthreads();
+  label_rows(g, lab, rows, planes, false);
   uint32_t *gl = a.lab + ib * g.V + gbase;
-  for (int i = tid; i < nsite; i += T) gl[i] = gbase + find_root(lab, i);
+  for (int i = tid; i < nsite; i += T) gl[i] = gbase + lab[i];
 }
 
 // z bonds between plane z0-1 (or Lz-1 for the periodic wrap) and plane z0,
@@ -628,10 +816,6 @@ __global__ void __launch_bounds__(256) k_hou_flatten(Lattice g, HouArgs a) {
   const uint32_t root = find_root(gl, i);
   gl[i] = root;
   if (a.labels_out) a.labels_out[ib * g.V + i] = root;
-  const uint64_t cidx = a.offset0 + (uint64_t)grp * g.V * g.nt * g.ns +
-                        (uint64_t)s * g.V * g.nt + (uint64_t)beta * g.V;
-  if (root == (uint32_t)i && coin(a.seed, cidx + i))
-    atomicOr(a.cbits + ib * (g.V / 32) + (i >> 5), 1u << (i & 31));
 }
 
 __global__ void __launch_bounds__(256) k_hou_apply(Lattice g, HouArgs a) {
@@ -663,8 +847,8 @@ inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) /
 // ------------------------------------------------------------- launchers
 
 hipError_t launch_sweep(const Lattice &g, const SweepArgs &a, hipStream_t s) {
-  dim3 grid(blocks_for((long long)g.n * g.nr, 256), g.nt);
-  hipLaunchKernelGGL(k_sweep, grid, dim3(256), 0, s, g, a);
+  dim3 grid(blocks_for((long long)g.n * g.nr, kSwPos), g.nt);
+  hipLaunchKernelGGL(k_sweep, grid, dim3(kSwThreads), 0, s, g, a);
   return hipGetLastError();
 }
 
@@ -683,21 +867,46 @@ hipError_t launch_pt(const Lattice &g, const PtArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+size_t hou_whole_lds(const Lattice &g) {
+  return 8 * (size_t)(g.V / g.L) + 32 * (size_t)g.n + 4 * (size_t)(g.V / 32) +
+         4 * (size_t)g.V;
+}
+
+size_t hou_slab_lds(const Lattice &g, int planes) {
+  return 8 * (size_t)planes * g.L + 4 * (size_t)planes * g.L * g.L;
+}
+
+static void allow_lds(const void *kernel, size_t bytes) {
+  // raise the dynamic LDS cap of a kernel once per process, to the largest
+  // size requested so far
+  static std::mutex mu;
+  static std::vector<std::pair<const void *, size_t>> done;
+  std::lock_guard<std::mutex> lock(mu);
+  for (auto &d : done)
+    if (d.first == kernel) {
+      if (d.second >= bytes) return;
+      d.second = bytes;
+      (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes);
+      return;
+    }
+  done.push_back({kernel, bytes});
+  (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 hipError_t launch_houdayer(const Lattice &g, const HouArgs &a, hipStream_t s) {
   if (g.ctm <= 0) return hipSuccess;
-  if (g.V <= 16384) {
-    hipLaunchKernelGGL(k_hou_whole<16384>, dim3(1, g.ctm, 2 * g.ns), dim3(1024), 0, s,
-                       g, a);
-    return hipGetLastError();
-  }
-  if (g.V <= kHouWholeMax) {
-    hipLaunchKernelGGL(k_hou_whole<kHouWholeMax>, dim3(1, g.ctm, 2 * g.ns), dim3(1024),
-                       0, s, g, a);
+  const dim3 inst(1, g.ctm, 2 * g.ns);
+  if (a.planes <= 0) {
+    const size_t lds = hou_whole_lds(g);
+    allow_lds(reinterpret_cast<const void *>(k_hou_whole), lds);
+    hipLaunchKernelGGL(k_hou_whole, inst, dim3(1024), lds, s, g, a);
     return hipGetLastError();
   }
   const int slabs = (g.Lz + a.planes - 1) / a.planes;
-  hipLaunchKernelGGL(k_hou_slab<kHouWholeMax>, dim3(slabs, g.ctm, 2 * g.ns), dim3(1024),
-                     0, s, g, a);
+  const size_t lds = hou_slab_lds(g, a.planes);
+  allow_lds(reinterpret_cast<const void *>(k_hou_slab), lds);
+  hipLaunchKernelGGL(k_hou_slab, dim3(slabs, g.ctm, 2 * g.ns), dim3(1024), lds, s, g, a);
   hipLaunchKernelGGL(k_hou_seam, dim3(slabs, g.ctm, 2 * g.ns), dim3(256), 0, s, g, a);
   hipLaunchKernelGGL(k_hou_flatten, dim3(blocks_for(g.V, 256), g.ctm, 2 * g.ns),
                      dim3(256), 0, s, g, a);

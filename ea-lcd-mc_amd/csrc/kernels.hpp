@@ -45,13 +45,15 @@ struct PtArgs {
 
 struct HouArgs {
   uint64_t *spin;
+  const uint64_t *J;     // [ns][2][n][6]
+  int32_t *ener;         // [ns][nr][nt] by clone; updated by the whole-tube path
   const uint8_t *wctb;
   int32_t pairs[4];
   uint64_t seed, offset0;  // coin index of (group 0, sample 0, beta 0, site 0)
   uint32_t *lab;         // [2][ns][ctm][V] global labels (tiled path)
   uint32_t *cbits;       // [2][ns][ctm][V/32] root coins (tiled path)
   uint32_t *labels_out;  // optional copy of the final labels
-  int planes;            // planes per tile (tiled path)
+  int planes;            // z-planes per slab; 0: the whole tube in one block
 };
 
 struct MeasureArgs {
@@ -72,7 +74,12 @@ hipError_t launch_measure(const Lattice &g, const MeasureArgs &a,
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
 
-// Largest lattice labelled entirely in one workgroup's LDS.
-constexpr int kHouWholeMax = 32768;
+// LDS bytes of the whole-tube and slab Houdayer kernels; a tube is labelled
+// in one workgroup when hou_whole_lds() fits the device's per-block LDS,
+// otherwise in slabs of about kHouSlabSites sites.  Both need L <= 64.
+size_t hou_whole_lds(const Lattice &g);
+size_t hou_slab_lds(const Lattice &g, int planes);
+constexpr int kHouSlabSites = 16384;
+constexpr int kHouMaxL = 64;
 
 }  // namespace eatube

@@ -871,13 +871,16 @@ void orc_labels(int L, int Lz, int bc, const uint8_t *q, uint32_t *label) {
 /* Cluster coin.  The reference draws cuRAND XORWOW doubles
  * (HoudayerXEA.cu:174-178,189,373): coin[beta][site] = U(idx) < 0.5 with
  * idx = offset + s*V*NT + beta*V + site.  That generator is closed; this
- * build keeps the index scheme and draws U from Philox4x32-10 with counter
- * (idx_lo, idx_hi, 0x486f7564, 0) and key (seed_lo, seed_hi). */
+ * build keeps the index scheme and takes the coin as bit (idx mod 128) of
+ * Philox4x32-10 at counter (b_lo, b_hi, 0x486f7564, 0), b = idx / 128, key
+ * (seed_lo, seed_hi) -- word (idx mod 128)/32, bit idx mod 32. */
 int orc_coin(uint64_t seed, uint64_t idx) {
-  uint32_t ctr[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), 0x486f7564u, 0u};
+  const uint64_t b = idx >> 7;
+  uint32_t ctr[4] = {(uint32_t)b, (uint32_t)(b >> 32), 0x486f7564u, 0u};
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)}, o[4];
   orc_philox(ctr, key, o);
-  return o[0] < 0x80000000u;
+  const unsigned j = (unsigned)(idx & 127);
+  return (int)((o[j >> 5] >> (j & 31)) & 1u);
 }
 
 /* site -> (parity, half index, bit) of the MSC layout (ini.c:355-375) */
@@ -967,6 +970,7 @@ int orc_sim_init(orc_sim *s, const orc_cfg *c) {
   s->acc = calloc((size_t)ns * nr * (nt - 1), sizeof(double));
   s->att = calloc((size_t)ns * nr * (nt - 1), sizeof(double));
   s->raw = calloc((size_t)ns * nr * (nt - 1), sizeof(double));
+  s->esum = calloc((size_t)ns * nr * nt, sizeof(double));
   s->rw = malloc(sizeof(uint64_t) * 3 * half);
   s->iter = 0;
   return 0;
@@ -990,6 +994,7 @@ void orc_sim_free(orc_sim *s) {
   free(s->acc);
   free(s->att);
   free(s->raw);
+  free(s->esum);
   free(s->rw);
   free(s->last_labels);
   memset(s, 0, sizeof *s);
@@ -1108,6 +1113,12 @@ void orc_sim_steps(orc_sim *s, int nsteps, int ptfr, int houfr, int cluster) {
     }
     if (cluster) orc_sim_houdayer(s);
     orc_sim_energy(s);
+    /* average_Energy accumulation, main.cu:209-218 */
+    const int nr = s->c.nr, nt = s->c.nt;
+    for (int k = 0; k < s->c.ns * nr; k++)
+      for (int b = 0; b < nt; b++)
+        s->esum[(size_t)k * nt + b] +=
+            (double)s->ener[(size_t)k * nt + s->wctb[(size_t)k * nt + b]];
   }
 }
 
@@ -1147,6 +1158,7 @@ void *orc_sim_field(orc_sim *s, const char *name, size_t *nbytes) {
   F("acc", s->acc, 8 * ns * nr * (nt - 1));
   F("att", s->att, 8 * ns * nr * (nt - 1));
   F("raw", s->raw, 8 * ns * nr * (nt - 1));
+  F("esum", s->esum, 8 * ns * nr * nt);
   F("pt", s->pt, sizeof(orc_hq) * ns * nr);
   F("prc", s->prc, sizeof(orc_hq) * nr);
   F("xo", s->xo, sizeof(orc_xo) * nr);

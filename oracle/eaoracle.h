@@ -191,6 +191,7 @@ typedef struct {
   uint8_t *wbtc;        /* [ns][nr][nt] which_beta_this_clon */
   int32_t *ener;        /* [ns][nr][nt] by clone */
   double *acc, *att, *raw; /* [ns][nr][nt-1] */
+  double *esum;         /* [ns][nr][nt] by beta: per-step energy sums */
   uint64_t *rw;         /* scratch random words */
   int iter;
   int last_pairs[4];    /* r1,r2 of group 0 and 1 of the last Houdayer call */

@@ -235,6 +235,7 @@ class Sim:
     FIELDS = {"spin": np.uint64, "J": np.uint64, "J6": np.uint8,
               "wctb": np.uint8, "wbtc": np.uint8, "ener": np.int32,
               "acc": np.float64, "att": np.float64, "raw": np.float64,
+              "esum": np.float64,
               "pt": np.uint8, "prc": np.uint8, "xo": np.uint8,
               "keys": np.uint32, "seeds_J": np.uint64, "seeds_u": np.uint64,
               "seeds_MC": np.uint64, "cl": np.uint8, "iter": np.int32,

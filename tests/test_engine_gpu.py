@@ -47,6 +47,9 @@ def assert_same_state(o, e, what=""):
     assert np.array_equal(acc.ravel(), o.field("acc")), what + " acceptance"
     assert np.array_equal(att.ravel(), o.field("att")), what + " attempts"
     assert np.array_equal(raw.ravel(), o.field("raw")), what + " raw"
+    es, _ = e.engine.read_measures()
+    assert np.array_equal(es.ravel(), o.field("esum")), what + " energy sums"
+    o.field("esum")[:] = 0
     o.energy()
     assert np.array_equal(e.engine.energy().ravel(), o.field("ener")), \
         what + " energies"
@@ -54,6 +57,17 @@ def assert_same_state(o, e, what=""):
 
 def test_device_present():
     assert eatube.device_count() >= 1
+    try:
+        import torch
+        p = torch.cuda.get_device_properties(0)
+        say("device:", p.name, getattr(p, "gcnArchName", ""),
+            "CUs", p.multi_processor_count,
+            "LDS/block", getattr(p, "shared_memory_per_block", None),
+            "LDS/block optin", getattr(p, "shared_memory_per_block_optin",
+                                       None),
+            "LDS/CU", getattr(p, "shared_memory_per_multiprocessor", None))
+    except Exception as ex:  # informational only
+        say("device properties unavailable:", ex)
 
 
 @pytest.mark.parametrize("case", [
@@ -88,8 +102,6 @@ def test_plain_small_config_with_pt():
         e.steps(chunk, cluster=False)
         o.steps(chunk, cluster=False)
         assert_same_state(o, e, "after %d steps" % chunk)
-    es, _ = e.engine.read_measures()
-    assert np.all(np.isfinite(es))
 
 
 @pytest.mark.parametrize("case", [
@@ -254,3 +266,14 @@ def test_full_size_pt_matches_oracle_pt():
         assert np.array_equal(a[1], att2[s, r])
         assert np.array_equal(a[2], raw2[s, r])
     assert bytes(ost) == bytes(st2)
+
+
+def test_pt_many_temperatures():
+    """220 temperatures: one PT call draws 219 numbers, so the generator's
+    256-slot ring wraps within the call."""
+    o, e = pair(8, 8, 2, 220, 2, bc=0, tmin=0.4, tmax=2.0,
+                seeds=(61, 62, 63, 64))
+    for chunk in (1, 3):
+        e.steps(chunk, cluster=False)
+        o.steps(chunk, cluster=False)
+        assert_same_state(o, e, "after %d steps" % chunk)
