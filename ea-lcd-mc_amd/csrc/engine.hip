@@ -61,6 +61,10 @@ struct ea_engine {
   uint8_t *wctb = nullptr;    // [ns][nr][nt]
   uint8_t *lut = nullptr;     // [nt][EA_LUT_BYTES]
   int32_t *ener = nullptr;    // [ns][nr][nt]
+  int32_t *part = nullptr;    // partial energies (see energy_partials)
+  // energy bookkeeping: the last sweep left partials by beta that no kernel
+  // has summed yet (part_pending), or ener holds the current energies
+  bool part_pending = false, ener_valid = false;
   ea_hq_state *pt = nullptr;  // [ns][nr]
   double *acc = nullptr, *att = nullptr, *raw = nullptr;
   double *nd = nullptr, *tab = nullptr;
@@ -100,7 +104,7 @@ struct ea_engine {
       (void)hipEventDestroy(m.b);
     }
     for (auto ev : spare) (void)hipEventDestroy(ev);
-    void *ptrs[] = {spin, J,   wctb, lut,  ener, pt,   acc,  att, raw,
+    void *ptrs[] = {spin, J,   wctb, lut,  ener, part, pt,   acc,  att, raw,
                     nd,   tab, toff, tlen, hlab, hcb,  keep, esum, hist};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -178,6 +182,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   g.bc = cfg->bc;
   g.ctm = cfg->clone_threshold;
   g.half = (int64_t)g.n * g.nr * g.nt;
+  set_divisors(g);
   e->job = cfg->job_id;
   e->betas.assign(cfg->betas, cfg->betas + cfg->nt);
 
@@ -188,6 +193,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
       (rc = dalloc(&e->wctb, chains * g.nt)) ||
       (rc = dalloc(&e->lut, (size_t)g.nt * EA_LUT_BYTES)) ||
       (rc = dalloc(&e->ener, chains * g.nt)) || (rc = dalloc(&e->pt, chains)) ||
+      (rc = dalloc(&e->part, energy_partials(g))) ||
       (rc = dalloc(&e->acc, chains * (g.nt - 1))) ||
       (rc = dalloc(&e->att, chains * (g.nt - 1))) ||
       (rc = dalloc(&e->raw, chains * (g.nt - 1))) ||
@@ -276,6 +282,7 @@ extern "C" int ea_upload_sample(ea_engine *e, int s, const uint64_t *even,
         t[((size_t)p * g.n + h) * 6 + d] = J[((size_t)p * 6 + d) * g.n + h];
   HIP_TRY(hipMemcpy(e->J + (size_t)s * 12 * g.n, t.data(), sizeof(uint64_t) * t.size(),
                     hipMemcpyHostToDevice));
+  e->part_pending = e->ener_valid = false;
   return EA_OK;
 }
 
@@ -303,6 +310,7 @@ extern "C" int ea_upload_perms(ea_engine *e, const uint8_t *wctb) {
   }
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipMemcpy(e->wctb, wctb, (size_t)g.ns * g.nr * g.nt, hipMemcpyHostToDevice));
+  e->part_pending = e->ener_valid = false;  // partials by beta map through the old perms
   return EA_OK;
 }
 
@@ -367,24 +375,52 @@ static int enqueue_sweeps(ea_engine *e, int nsweeps, const ea_counter *blocks,
   a.J = e->J;
   a.wctb = e->wctb;
   a.lut = e->lut;
-  a.ener = e->ener;
+  a.part = e->part;
   std::memcpy(a.keys, e->keys, sizeof a.keys);
   const int nr = e->g.nr;
   for (int sw = 0; sw < nsweeps; ++sw) {
     const bool last = energy && sw == nsweeps - 1;
     for (int p = 0; p < 2; ++p) {
       a.parity = p;
-      a.energy_mode = last ? (p == 0 ? 1 : 2) : 0;
+      a.energy = last && p == 1;
       std::memcpy(a.ctr, blocks + ((size_t)sw * 2 + p) * nr, sizeof(ea_counter) * nr);
       Timed t(e, EA_K_SWEEP);
       HIP_TRY(launch_sweep(e->g, a, e->stream));
     }
   }
+  if (nsweeps > 0) {
+    e->part_pending = energy;
+    e->ener_valid = false;
+  }
   return EA_OK;
 }
 
-static int enqueue_pt(ea_engine *e) {
+// Makes ener hold the current energies by clone: sums the last sweep's
+// partials, or recomputes them from the spins.
+static int ensure_energy(ea_engine *e) {
+  if (e->part_pending) {
+    Timed t(e, EA_K_ENERGY);
+    HIP_TRY(launch_resolve(e->g, e->part, e->wctb, e->ener, e->stream));
+  } else if (!e->ener_valid) {
+    Timed t(e, EA_K_ENERGY);
+    HIP_TRY(launch_energy(e->g, e->spin, e->J, e->ener, e->part, e->stream));
+  }
+  e->part_pending = false;
+  e->ener_valid = true;
+  return EA_OK;
+}
+
+// measure: add the per-beta energies after the swaps to esum and, for
+// row >= 0, write that history row.
+static int enqueue_pt(ea_engine *e, bool measure = false, int row = -1) {
   PtArgs a{};
+  if (e->part_pending) {
+    a.part = e->part;
+    a.nbx = sweep_blocks_x(e->g);
+  } else {
+    const int rc = ensure_energy(e);
+    if (rc) return rc;
+  }
   a.st = e->pt;
   a.ener = e->ener;
   a.wctb = e->wctb;
@@ -395,13 +431,21 @@ static int enqueue_pt(ea_engine *e) {
   a.tab = e->tab;
   a.off = e->toff;
   a.len = e->tlen;
+  a.esum = e->esum;
+  a.hist = e->hist;
+  a.measure = measure;
+  a.row = row;
   Timed t(e, EA_K_PT);
   HIP_TRY(launch_pt(e->g, a, e->stream));
+  e->part_pending = false;
+  e->ener_valid = true;
   return EA_OK;
 }
 
+// measure (whole-tube path only): correct the per-beta sums a measuring PT
+// added just before for the configurations the move changes.
 static int enqueue_houdayer(ea_engine *e, const int32_t pairs[4], uint64_t seed,
-                            uint64_t offset) {
+                            uint64_t offset, bool measure = false) {
   const int half = e->g.nr / 2;
   for (int k = 0; k < 4; ++k) {
     const int lo = (k / 2) * half;
@@ -410,6 +454,10 @@ static int enqueue_houdayer(ea_engine *e, const int32_t pairs[4], uint64_t seed,
   }
   if (pairs[0] == pairs[1] || pairs[2] == pairs[3])
     return fail(EA_E_ARG, "a Houdayer pair needs two distinct replicas");
+  if (e->planes == 0) {  // the whole-tube kernel updates the energies it moves
+    const int rc = ensure_energy(e);
+    if (rc) return rc;
+  }
   HouArgs a{};
   a.spin = e->spin;
   a.J = e->J;
@@ -422,8 +470,11 @@ static int enqueue_houdayer(ea_engine *e, const int32_t pairs[4], uint64_t seed,
   a.cbits = e->hcb;
   a.labels_out = e->keep_on ? e->keep : nullptr;
   a.planes = e->planes;
+  a.esum = e->esum;
+  a.measure = measure && e->planes == 0;
   Timed t(e, EA_K_HOUDAYER);
   HIP_TRY(launch_houdayer(e->g, a, e->stream));
+  if (e->planes > 0) e->part_pending = e->ener_valid = false;
   return EA_OK;
 }
 
@@ -438,8 +489,10 @@ extern "C" int ea_energy(ea_engine *e, int32_t *out) {
   const Lattice &g = e->g;
   {
     Timed t(e, EA_K_ENERGY);
-    HIP_TRY(launch_energy(g, e->spin, e->J, e->ener, e->stream));
+    HIP_TRY(launch_energy(g, e->spin, e->J, e->ener, e->part, e->stream));
   }
+  e->part_pending = false;
+  e->ener_valid = true;
   if (out) {
     HIP_TRY(hipMemcpyAsync(out, e->ener, sizeof(int32_t) * g.ns * g.nr * g.nt,
                            hipMemcpyDeviceToHost, e->stream));
@@ -505,37 +558,43 @@ extern "C" int ea_run(ea_engine *e, const ea_schedule *sch, uint64_t *rng_offset
   const uint64_t per_call = 2ull * g.V * g.nt * g.ns;
   const size_t blocks_per_sweep = 2 * (size_t)g.nr;
   size_t bi = 0;
+  // The step's energy measure (main.cu:208-228) rides on its last PT, which
+  // adds the energies after the swaps; the whole-tube cluster move then
+  // corrects the sums for what it changed.  The slab path recomputes the
+  // energies and measures separately.
+  const bool fused = !sch->cluster || e->planes == 0;
   for (int it = 0; it < sch->n_it; ++it) {
+    const int itb = sch->it0 + it;
+    const int row = (itb % sch->frec_trw == sch->frec_trw - 1) ? itb / sch->frec_trw : -1;
     for (int im = 0; im < sch->mesfr; ++im) {
+      const bool last_m = im == sch->mesfr - 1;
       for (int ih = 0; ih < sch->houfr; ++ih) {
         int rc = enqueue_sweeps(e, sch->ptfr, sch->blocks + bi * blocks_per_sweep, true);
         if (rc) return rc;
         bi += sch->ptfr;
-        if ((rc = enqueue_pt(e))) return rc;
+        const bool measure = fused && last_m && ih == sch->houfr - 1;
+        if ((rc = enqueue_pt(e, measure, measure ? row : -1))) return rc;
       }
       if (sch->cluster) {
         const int rc = enqueue_houdayer(
-            e, sch->pairs + ((size_t)it * sch->mesfr + im) * 4, sch->coin_seed, offset);
+            e, sch->pairs + ((size_t)it * sch->mesfr + im) * 4, sch->coin_seed, offset,
+            fused && last_m);
         if (rc) return rc;
         offset += per_call;
       }
     }
-    // the whole-tube cluster move updates the energies it changes; the slab
-    // path leaves them to a full energy pass; without cluster moves the
-    // sweep's energies are current (PT only permutes clones)
-    if (sch->cluster && e->planes > 0) {
-      Timed t(e, EA_K_ENERGY);
-      HIP_TRY(launch_energy(g, e->spin, e->J, e->ener, e->stream));
+    if (!fused) {
+      int rc = ensure_energy(e);
+      if (rc) return rc;
+      MeasureArgs m{};
+      m.ener = e->ener;
+      m.wctb = e->wctb;
+      m.esum = e->esum;
+      m.hist = e->hist;
+      m.row = row;
+      Timed t(e, EA_K_MEASURE);
+      HIP_TRY(launch_measure(g, m, e->stream));
     }
-    MeasureArgs m{};
-    m.ener = e->ener;
-    m.wctb = e->wctb;
-    m.esum = e->esum;
-    m.hist = e->hist;
-    const int itb = sch->it0 + it;
-    m.row = (itb % sch->frec_trw == sch->frec_trw - 1) ? itb / sch->frec_trw : -1;
-    Timed t(e, EA_K_MEASURE);
-    HIP_TRY(launch_measure(g, m, e->stream));
   }
   if (rng_offset_out) *rng_offset_out = offset;
   return EA_OK;

@@ -212,8 +212,6 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(Lattice g, SweepArgs a) {
     ctr[tx][3] = a.ctr[tx].w;
     ctr[tx][4] = a.keys[tx];
   }
-  if (a.energy_mode == 1 && blockIdx.x == 0 && blockIdx.y == 0)
-    for (int i = tx; i < g.ns * g.nr * g.nt; i += kSwThreads) a.ener[i] = 0;
   __syncthreads();
 
   const int nwords = g.n * g.nr, p0 = blockIdx.x * kSwPos;
@@ -245,7 +243,6 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(Lattice g, SweepArgs a) {
   const uint64_t id1 = ids[pos][0], id2 = ids[pos][1];
   const int p = a.parity;
   const Nbrs nb = neighbours(g, p, h);
-  const bool wave_live = p0 < nwords;
 
   for (int s = slot; s < g.ns; s += kSwSlots) {
     const int clone = clone_at[s * g.nr + r];
@@ -271,18 +268,19 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(Lattice g, SweepArgs a) {
       const uint64_t flip = (j1 & j3 & id1) | ((j24 | id2) & (j1 | j3 | id1)) |
                             (j2 & j4) | (j24 & id2);
       cur[(size_t)h * g.nr + r] = sv ^ flip;
-      if (a.energy_mode == 2) {
+      if (a.energy) {
         int u = 0;
 #pragma unroll
         for (int d = 0; d < 6; ++d) u += __popcll(f[d] ^ flip);
         e = 6 * 64 - 2 * u;
       }
     }
-    if (a.energy_mode == 2) {
-      // every bond has exactly one odd end: the odd half-sweep sees them all
+    if (a.energy) {
+      // every bond has exactly one odd end: the odd half-sweep sees them
+      // all; one partial per (sample, replica, beta, block), no atomics
       e = replica_sum(e, g.nr);
-      if (wave_live && pos < g.nr)
-        atomicAdd(a.ener + ((size_t)s * g.nr + r) * g.nt + clone, e);
+      if (pos < g.nr)
+        a.part[(((size_t)s * g.nr + r) * g.nt + beta) * gridDim.x + blockIdx.x] = e;
     }
   }
 }
@@ -291,8 +289,9 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(Lattice g, SweepArgs a) {
 
 // Energy of every (sample, replica, clone) from the even words
 // (d_computeEne, main.cu:1209-1381): sum of 384 - 2 popcount(unsatisfied).
+// Each wave writes one partial per replica; k_energy_sum adds them up.
 __global__ void __launch_bounds__(256) k_energy(Lattice g, const uint64_t *spin,
-                                                const uint64_t *J, int32_t *ener) {
+                                                const uint64_t *J, int32_t *part) {
   const int clone = blockIdx.y, s = blockIdx.z;
   const int nwords = g.n * g.nr;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -312,9 +311,49 @@ __global__ void __launch_bounds__(256) k_energy(Lattice g, const uint64_t *spin,
     e = 6 * 64 - 2 * u;
   }
   e = replica_sum(e, g.nr);
-  if ((int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)) < nwords &&
-      (threadIdx.x & 63) < g.nr)
-    atomicAdd(ener + ((size_t)s * g.nr + r) * g.nt + clone, e);
+  const int lane = threadIdx.x & 63, nw = gridDim.x * (blockDim.x / 64);
+  const int wv = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (lane < g.nr) part[(((size_t)s * g.nr + r) * g.nt + clone) * nw + wv] = e;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+  for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum of the nw partials of each of a chain's nt slots into out[slot]: one
+// lane per slot when the partials are few, the whole wave per slot otherwise.
+__device__ __forceinline__ void sum_partials(const int32_t *part, int nt, int nw,
+                                             int lane, int32_t *out) {
+  if (nw <= 32) {
+    for (int b = lane; b < nt; b += 64) {
+      int acc = 0;
+      for (int k = 0; k < nw; ++k) acc += part[(size_t)b * nw + k];
+      out[b] = acc;
+    }
+  } else {
+    for (int b = 0; b < nt; ++b) {
+      int acc = 0;
+      for (int k = lane; k < nw; k += 64) acc += part[(size_t)b * nw + k];
+      acc = wave_sum(acc);
+      if (lane == 0) out[b] = acc;
+    }
+  }
+}
+
+// ener[chain][c] from the chain's partials: slot c itself (by_beta NULL),
+// or slot b whose clone is c = by_beta[chain][b] (the sweep's partials).
+__global__ void __launch_bounds__(64) k_energy_sum(Lattice g, const int32_t *part,
+                                                   int nw, const uint8_t *by_beta,
+                                                   int32_t *ener) {
+  __shared__ int32_t E[EA_MAX_BETAS];
+  const int chain = blockIdx.x, nt = g.nt, lane = threadIdx.x;
+  sum_partials(part + (size_t)chain * nt * nw, nt, nw, lane, E);
+  __syncthreads();
+  for (int b = lane; b < nt; b += 64) {
+    const int c = by_beta ? by_beta[(size_t)chain * nt + b] : b;
+    ener[(size_t)chain * nt + c] = E[b];
+  }
 }
 
 // ------------------------------------------------------ parallel tempering
@@ -350,9 +389,19 @@ __global__ void __launch_bounds__(64) k_pt(Lattice g, PtArgs a) {
   const int chain = blockIdx.x, nt = g.nt, np = nt - 1, lane = threadIdx.x;
   ea_hq_state *st = a.st + chain;
   for (int i = lane; i < 256; i += 64) ira[i] = st->ira[i];
-  for (int i = lane; i < nt; i += 64) {
-    E[i] = a.ener[(size_t)chain * nt + i];
-    perm[i] = a.wctb[(size_t)chain * nt + i];
+  for (int i = lane; i < nt; i += 64) perm[i] = a.wctb[(size_t)chain * nt + i];
+  if (a.part) {
+    // the sweep left partial energies by beta: dE holds them by beta for a
+    // moment, E receives them by clone
+    sum_partials(a.part + (size_t)chain * nt * a.nbx, nt, a.nbx, lane, dE);
+    __syncthreads();
+    for (int b = lane; b < nt; b += 64) {
+      E[perm[b]] = dE[b];
+      a.ener[(size_t)chain * nt + perm[b]] = dE[b];
+    }
+    __syncthreads();
+  } else {
+    for (int c = lane; c < nt; c += 64) E[c] = a.ener[(size_t)chain * nt + c];
   }
   const uint8_t ip = st->ip;
   if (lane == 0) {  // the congruential half is a cheap serial recurrence
@@ -418,6 +467,12 @@ __global__ void __launch_bounds__(64) k_pt(Lattice g, PtArgs a) {
   __syncthreads();
   for (int i = lane; i < 256; i += 64) st->ira[i] = ira[i];
   for (int i = lane; i < nt; i += 64) a.wctb[(size_t)chain * nt + i] = perm[i];
+  if (a.measure)  // per-beta energy sums and history (main.cu:208-228)
+    for (int b = lane; b < nt; b += 64) {
+      const int c = perm[b];
+      a.esum[(size_t)chain * nt + b] += (double)E[c];
+      if (a.row >= 0) a.hist[((size_t)a.row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
+    }
   for (int i = lane; i < np; i += 64) {
     const size_t k = (size_t)chain * np + i;
     const double pt = a.nd[i] * (double)dE[i];
@@ -545,17 +600,95 @@ __device__ __forceinline__ void unite(uint32_t *lab, uint32_t a, uint32_t b) {
   }
 }
 
-__device__ __forceinline__ uint32_t qbit(const uint32_t *qb, int i) {
-  return (qb[i >> 5] >> (i & 31)) & 1u;
+// Quotient i / d from m = ceil(2^32 / d) (m = 0 for d = 1): exact for
+// i < 2^26 and d <= 64, one multiply instead of a division sequence.
+__device__ __forceinline__ uint32_t qdiv(uint32_t i, uint32_t m) {
+  return m ? __umulhi(i, m) : i;
 }
+
+// Union-find forest in LDS, parents as 32-bit node indices.  Loads and stores
+// are relaxed workgroup-scope atomics: every step re-reads the parent from
+// LDS, and the accesses stay LDS instructions.  Every parent index is below
+// its child's, so the forest has no cycles; a node that is not a root only
+// ever gets re-pointed at an ancestor.
+struct Forest {
+  uint32_t *p;
+  __device__ __forceinline__ uint32_t get(uint32_t i) const {
+    return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ __forceinline__ void put(uint32_t i, uint32_t v) const {
+    __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ __forceinline__ uint32_t root(uint32_t a) const {
+    uint32_t q = get(a);
+    while (q != a) {
+      a = q;
+      q = get(a);
+    }
+    return a;
+  }
+  // root with path splitting (each node passed is pointed at its grandparent)
+  __device__ __forceinline__ uint32_t find(uint32_t a) const {
+    uint32_t q = get(a);
+    while (q != a) {
+      const uint32_t gq = get(q);
+      if (gq != q) put(a, gq);
+      a = q;
+      q = gq;
+    }
+    return a;
+  }
+  // Komura's union: the larger root goes under the smaller one; a link
+  // displaced by a concurrent union is made again from the displaced parent.
+  __device__ __forceinline__ void unite(uint32_t a, uint32_t b) const {
+    a = find(a);
+    b = find(b);
+    while (a != b) {
+      if (a < b) {
+        const uint32_t t = a;
+        a = b;
+        b = t;
+      }
+      const uint32_t old = atomicMin(p + a, b);
+      if (old == a) return;
+      a = old;
+    }
+  }
+};
+
+// Rows of L <= 64 sites as bit masks (bit x = site x); RowT is 32 bits wide
+// when L <= 32.
+template <typename RowT>
+struct Rows {
+  static __device__ __forceinline__ int hi(RowT v) {
+    if constexpr (sizeof(RowT) == 4) return 31 - __clz(v);
+    else return 63 - __clzll(v);
+  }
+  static __device__ __forceinline__ int lo(RowT v) {
+    if constexpr (sizeof(RowT) == 4) return __ffs(v) - 1;
+    else return __ffsll((unsigned long long)v) - 1;
+  }
+  // run boundaries of the periodic row: bit x where q[x] != q[x-1 mod L]
+  static __device__ __forceinline__ RowT bounds(RowT q, int L, RowT full) {
+    return (q ^ ((q << 1) | (q >> (L - 1)))) & full;
+  }
+  // first site of the run holding x, 0 for the run that wraps past L-1 (and
+  // for a constant row): the smallest x of the run
+  static __device__ __forceinline__ int rep(RowT b, int x) {
+    const RowT m = b & ((RowT(2) << x) - 1);
+    if (m == 0) return 0;
+    const int s = hi(m);
+    return (!(b & 1) && s == hi(b)) ? 0 : s;
+  }
+};
 
 // Cluster labelling of a slab of `planes` full z-planes in LDS, all threads
 // of the block taking part.  rows[r] holds the q bits of row r = y + L*z
 // (bit x; L <= 64); lab[i] is the label of local site i = x + L*r.  Label
 // values are local site indices and end as the smallest index of each
 // equal-q cluster:
-//  1. runs of equal q along the periodic x row point at their first site
-//     (a run wrapping past x = L-1 points at the row start); no atomics;
+//  1. every site points at the first site of its run of equal q along the
+//     periodic x row (a run wrapping past x = L-1 at the row start);
 //  2. y and z bonds between rows, from bit masks: a bond at x is united
 //     only where a run overlap starts (when the bond at x-1 joins the same
 //     two runs the union is implied);
@@ -563,31 +696,27 @@ __device__ __forceinline__ uint32_t qbit(const uint32_t *qb, int i) {
 // z bonds leave the slab only when wrap_z (the whole periodic tube).
 __device__ void label_rows(const Lattice &g, uint32_t *lab, const uint64_t *rows,
                            int planes, bool wrap_z) {
+  using R = Rows<uint64_t>;
+  const Forest f{lab};
   const int L = g.L, nrows = planes * L, T = blockDim.x, tid = threadIdx.x;
   const uint64_t full = L == 64 ? ~0ULL : ((1ULL << L) - 1);
   for (int i = tid; i < nrows * L; i += T) {
-    const int r = i / L, x = i - r * L, base = i - x;
-    const uint64_t q = rows[r];
-    const uint64_t bnd = ((q ^ (q << 1)) & full) | 1ULL;  // run starts
-    int start = 63 - __clzll(bnd & ((2ULL << x) - 1));
-    // a last run continuing through the periodic wrap joins the first
-    const int last = 63 - __clzll(bnd);
-    if (last != 0 && start == last && !(((q >> (L - 1)) ^ q) & 1)) start = 0;
-    lab[i] = base + start;
+    const int r = qdiv(i, g.mL), x = i - r * L;
+    lab[i] = r * L + R::rep(R::bounds(rows[r], L, full), x);
   }
   __syncthreads();
   for (int r = tid; r < nrows; r += T) {
     const uint64_t q = rows[r];
     const uint64_t lq = ~(q ^ (q << 1)) & full & ~1ULL;  // q[x] == q[x-1]
-    const int y = r % L, zl = r / L;
+    const int zl = qdiv(r, g.mL), y = r - zl * L;
     auto link = [&](int r2) {
       const uint64_t q2 = rows[r2];
       const uint64_t l2 = ~(q2 ^ (q2 << 1)) & full & ~1ULL;
       uint64_t todo = ~(q ^ q2) & full & ~(lq & l2);
       while (todo) {
-        const int x = __ffsll((unsigned long long)todo) - 1;
+        const int x = R::lo(todo);
         todo &= todo - 1;
-        unite(lab, r * L + x, r2 * L + x);
+        f.unite(r * L + x, r2 * L + x);
       }
     };
     link(y + 1 == L ? r - (L - 1) : r + 1);
@@ -597,46 +726,43 @@ __device__ void label_rows(const Lattice &g, uint32_t *lab, const uint64_t *rows
       link(r - L * (planes - 1));
   }
   __syncthreads();
-  for (int i = tid; i < nrows * L; i += T) lab[i] = find_root(lab, i);
+  for (int i = tid; i < nrows * L; i += T) lab[i] = f.root(i);
   __syncthreads();
 }
 
-// q bits of row (y, z) from the two replicas' words: the L sites of a row
-// use ML words (parity alternating with x), each for its 4 x copies (bx).
+// q bits of row r = y + L z from the two replicas' words: the L sites of a
+// row use ML words (parity alternating with x), each for its 4 x copies (bx).
 template <typename Load>
-__device__ __forceinline__ uint64_t row_bits(const Lattice &g, int y, int z,
-                                             Load load) {
-  const int by = y / g.ML, yb = y - by * g.ML, bz = z / g.MLz, zg = z - bz * g.MLz;
+__device__ __forceinline__ uint64_t row_bits(const Lattice &g, int r, Load load) {
+  const int z = qdiv(r, g.mL), y = r - z * g.L;
+  const int by = qdiv(y, g.mML), yb = y - by * g.ML;
+  const int bz = qdiv(z, g.mMLz), zg = z - bz * g.MLz;
   const int hrow = g.ML * yb + g.MS * zg, bitrow = 4 * by + 16 * bz;
   const int prow = (y ^ z) & 1;
   uint64_t bits = 0;
   for (int ax = 0; ax < g.ML; ++ax) {
     const uint64_t q = load((ax + prow) & 1, (ax + hrow) >> 1) >> bitrow;
 #pragma unroll
-    for (int bx = 0; bx < 4; ++bx)
-      bits |= ((q >> bx) & 1ULL) << (bx * g.ML + ax);
+    for (int bx = 0; bx < 4; ++bx) bits |= ((q >> bx) & 1ULL) << (bx * g.ML + ax);
   }
   return bits;
 }
 
-// Move bits of word (p, h) gathered from the linear move bitmap mb (bit i
-// of site i); 4 threads per word, thread `quarter` gathering the 16 copies
-// of z block bz = quarter.
-__device__ __forceinline__ uint64_t word_move_bits(const Lattice &g,
-                                                   const uint64_t *mb, int p,
-                                                   int h, int quarter) {
-  const int gz = h / g.hs, rem = h - gz * g.hs, yb = rem / g.hl;
+// Move bits of word (p, h) gathered from the per-row move masks mrow; 4
+// threads per word, thread `quarter` gathering the 16 copies of z block
+// bz = quarter.
+__device__ __forceinline__ uint64_t word_move_bits(const Lattice &g, const uint64_t *mrow,
+                                                   int p, int h, int quarter) {
+  const int gz = qdiv(h, g.mhs), rem = h - gz * g.hs, yb = qdiv(rem, g.mhl);
   const int ax = 2 * (rem - yb * g.hl) + ((p + yb + gz) & 1);
-  const int zoff = g.L * g.L * (quarter * g.MLz + gz);
+  const int z = quarter * g.MLz + gz;
   uint64_t m = 0;
 #pragma unroll
   for (int by = 0; by < 4; ++by) {
-    const int rowbase = zoff + g.L * (by * g.ML + yb) + ax;
+    const uint64_t row = mrow[(by * g.ML + yb) + g.L * z] >> ax;
 #pragma unroll
-    for (int bx = 0; bx < 4; ++bx) {
-      const int site = rowbase + bx * g.ML;
-      m |= ((mb[site >> 6] >> (site & 63)) & 1ULL) << (bx + 4 * by + 16 * quarter);
-    }
+    for (int bx = 0; bx < 4; ++bx)
+      m |= ((row >> (bx * g.ML)) & 1ULL) << (bx + 4 * by + 16 * quarter);
   }
   return m;
 }
@@ -648,19 +774,25 @@ __device__ __forceinline__ uint64_t word_move_bits(const Lattice &g,
 // label = minimum linear site index, one coin per cluster root
 // (spin_select, :358-374), then q' = q ^ coin(label) and s1 ^= q', s2 ^= q'
 // (spin_flip_YK_custom + d_pack + d_useoverlap).
+//
+// The forest has one node per site, but only the first site of each run of
+// equal q along x (its representative, Rows::rep) takes part: a bond is
+// united between the representatives of its two ends, and each run reads
+// its cluster's root once.  A row's move mask is then built run by run.
 // The kernel also recomputes the energies of the two configurations it
-// changed (the Measure_Energy that follows a cluster move, main.cu:209), so
-// no separate energy pass is needed.
+// changed (the Measure_Energy that follows a cluster move, main.cu:209).
 // Dynamic LDS: rows[V/L] u64 | qw[2n] u64 | nw[2n] u64 | cb[V/32] u32 (coin
-// of every site) | lab[V] u32.
+// of every site) | forest[V] u32.
+template <typename RowT>
 __global__ void __launch_bounds__(1024) k_hou_whole(Lattice g, HouArgs a) {
+  using R = Rows<RowT>;
   extern __shared__ uint64_t lds[];
-  const int V = g.V, nrows = V / g.L, T = blockDim.x, tid = threadIdx.x;
-  uint64_t *rows = lds;
+  const int V = g.V, L = g.L, nrows = V / L, T = blockDim.x, tid = threadIdx.x;
+  uint64_t *rows = lds;         // q rows, then move rows
   uint64_t *qw = rows + nrows;  // q words, then replica r1's new words
   uint64_t *nw = qw + 2 * g.n;  // replica r2's new words
   uint32_t *cb = reinterpret_cast<uint32_t *>(nw + 2 * g.n);
-  uint32_t *lab = cb + V / 32;
+  const Forest f{cb + V / 32};
   __shared__ int esum[2];
   const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
   const Pair pw = pair_words(g, a, s, grp, beta);
@@ -670,24 +802,78 @@ __global__ void __launch_bounds__(1024) k_hou_whole(Lattice g, HouArgs a) {
     const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
     qw[w] = a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
   }
+  for (int i = tid; i < V; i += T) f.p[i] = i;
   __syncthreads();
   for (int r = tid; r < nrows; r += T)
-    rows[r] = row_bits(g, r % g.L, r / g.L,
-                       [&](int p, int h) { return qw[p * g.n + h]; });
+    rows[r] = row_bits(g, r, [&](int p, int h) { return qw[p * g.n + h]; });
   __syncthreads();
-  label_rows(g, lab, rows, g.Lz, g.bc == EA_PBC);
+
+  const RowT full = (RowT)(L == 64 ? ~0ULL : ((1ULL << L) - 1));
+  for (int r = tid; r < nrows; r += T) {
+    const RowT q = (RowT)rows[r], b = R::bounds(q, L, full);
+    const RowT lq = ~(q ^ (q << 1)) & full & ~RowT(1);  // q[x] == q[x-1]
+    const int zl = qdiv(r, g.mL), y = r - zl * L;
+    auto link = [&](int r2) {
+      const RowT q2 = (RowT)rows[r2], b2 = R::bounds(q2, L, full);
+      const RowT l2 = ~(q2 ^ (q2 << 1)) & full & ~RowT(1);
+      RowT todo = ~(q ^ q2) & full & ~(lq & l2);  // where a run overlap starts
+      while (todo) {
+        const int x = R::lo(todo);
+        todo &= todo - 1;
+        f.unite(r * L + R::rep(b, x), r2 * L + R::rep(b2, x));
+      }
+    };
+    link(y + 1 == L ? r - (L - 1) : r + 1);
+    if (zl + 1 < g.Lz)
+      link(r + L);
+    else if (g.bc == EA_PBC)
+      link(r - L * (g.Lz - 1));
+  }
+  // coins of 128 consecutive sites per Philox call
   const uint64_t cidx = a.offset0 + (uint64_t)grp * V * g.nt * g.ns +
                         (uint64_t)s * V * g.nt + (uint64_t)beta * V;
-  for (int c = tid; c < V / 32; c += T) cb[c] = coin_word(a.seed, cidx + 32 * c);
+  for (int c = tid; c < V / 128; c += T) {
+    const uint64_t blk = (cidx >> 7) + c;
+    uint32_t v0 = (uint32_t)blk, v1 = (uint32_t)(blk >> 32), v2 = 0x486f7564u, v3 = 0;
+    philox10(v0, v1, v2, v3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+    cb[4 * c] = v0;
+    cb[4 * c + 1] = v1;
+    cb[4 * c + 2] = v2;
+    cb[4 * c + 3] = v3;
+  }
   __syncthreads();
   if (a.labels_out) {
     uint32_t *out = a.labels_out + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
-    for (int i = tid; i < V; i += T) out[i] = lab[i];
+    for (int i = tid; i < V; i += T) {
+      const int r = qdiv(i, g.mL), x = i - r * L;
+      out[i] = f.root(r * L + R::rep(R::bounds((RowT)rows[r], L, full), x));
+    }
   }
-  for (int i = tid; i < V; i += T) {  // rows now receive the move bits,
-    const uint32_t root = lab[i];     // as a linear bitmap (64 sites/word)
-    const uint64_t m = __ballot((cb[root >> 5] >> (root & 31)) & 1u);
-    if ((tid & 63) == 0) rows[i >> 6] = m;
+  for (int r = tid; r < nrows; r += T) {  // move mask of row r, run by run
+    const RowT q = (RowT)rows[r], b = R::bounds(q, L, full);
+    const int base = r * L;
+    RowT m = 0;
+    if (b == 0) {
+      const uint32_t root = f.root(base);
+      m = ((cb[root >> 5] >> (root & 31)) & 1u) ? full : 0;
+    } else {
+      const int first = R::lo(b);
+      RowT bb = b;
+      while (bb) {
+        const int st = R::lo(bb);
+        bb &= bb - 1;
+        RowT seg = (bb ? (RowT(1) << R::lo(bb)) : RowT(0)) - (RowT(1) << st);
+        if (!bb) seg &= full;
+        int rp = st;
+        if (!bb && first != 0) {  // the last run wraps into the row start
+          rp = 0;
+          seg |= (RowT(1) << first) - 1;
+        }
+        const uint32_t root = f.root(base + rp);
+        if ((cb[root >> 5] >> (root & 31)) & 1u) m |= seg;
+      }
+    }
+    rows[r] = m;  // only this thread reads row r in this loop
   }
   __syncthreads();
   for (int t0 = 0; t0 < 8 * g.n; t0 += T) {  // every lane reaches the shuffles
@@ -738,8 +924,13 @@ __global__ void __launch_bounds__(1024) k_hou_whole(Lattice g, HouArgs a) {
   __syncthreads();
   if (tid < 2) {
     const int r = a.pairs[2 * grp + tid];
-    const int c = a.wctb[((size_t)s * g.nr + r) * g.nt + beta];
-    a.ener[((size_t)s * g.nr + r) * g.nt + c] = esum[tid];
+    const size_t chain = (size_t)s * g.nr + r;
+    const int c = a.wctb[chain * g.nt + beta];
+    const int old = a.ener[chain * g.nt + c];
+    a.ener[chain * g.nt + c] = esum[tid];
+    // the sums already hold the pre-move energy; all values are integers
+    // below 2^53, so adding the difference is exact
+    if (a.measure) a.esum[chain * g.nt + beta] += (double)(esum[tid] - old);
   }
 }
 
@@ -775,7 +966,7 @@ __global__ void __launch_bounds__(1024) k_hou_slab(Lattice g, HouArgs a) {
   const size_t ib = instance(g, grp, s, beta);
   const int gbase = z0 * S;
   for (int r = tid; r < planes * g.L; r += T)
-    rows[r] = row_bits(g, r % g.L, z0 + r / g.L, [&](int p, int h) {
+    rows[r] = row_bits(g, z0 * g.L + r, [&](int p, int h) {
       const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
       return a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
     });
@@ -846,19 +1037,34 @@ inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) /
 
 // ------------------------------------------------------------- launchers
 
+int sweep_blocks_x(const Lattice &g) { return (int)blocks_for((long long)g.n * g.nr, kSwPos); }
+
+size_t energy_partials(const Lattice &g) {
+  const size_t per = (size_t)g.ns * g.nr * g.nt;
+  const size_t sweep = per * sweep_blocks_x(g);
+  const size_t energy = per * 4 * blocks_for((long long)g.n * g.nr, 256);
+  return std::max(sweep, energy);
+}
+
 hipError_t launch_sweep(const Lattice &g, const SweepArgs &a, hipStream_t s) {
-  dim3 grid(blocks_for((long long)g.n * g.nr, kSwPos), g.nt);
+  dim3 grid(sweep_blocks_x(g), g.nt);
   hipLaunchKernelGGL(k_sweep, grid, dim3(kSwThreads), 0, s, g, a);
   return hipGetLastError();
 }
 
 hipError_t launch_energy(const Lattice &g, const uint64_t *spin, const uint64_t *J,
-                         int32_t *ener, hipStream_t s) {
-  hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t *>(ener), 0,
-                                 (size_t)g.ns * g.nr * g.nt, s);
-  if (e != hipSuccess) return e;
-  dim3 grid(blocks_for((long long)g.n * g.nr, 256), g.nt, g.ns);
-  hipLaunchKernelGGL(k_energy, grid, dim3(256), 0, s, g, spin, J, ener);
+                         int32_t *ener, int32_t *part, hipStream_t s) {
+  const unsigned bx = blocks_for((long long)g.n * g.nr, 256);
+  hipLaunchKernelGGL(k_energy, dim3(bx, g.nt, g.ns), dim3(256), 0, s, g, spin, J, part);
+  hipLaunchKernelGGL(k_energy_sum, dim3(g.ns * g.nr), dim3(64), 0, s, g,
+                     (const int32_t *)part, (int)(4 * bx), (const uint8_t *)nullptr, ener);
+  return hipGetLastError();
+}
+
+hipError_t launch_resolve(const Lattice &g, const int32_t *part, const uint8_t *wctb,
+                          int32_t *ener, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_sum, dim3(g.ns * g.nr), dim3(64), 0, s, g, part,
+                     sweep_blocks_x(g), wctb, ener);
   return hipGetLastError();
 }
 
@@ -899,8 +1105,13 @@ hipError_t launch_houdayer(const Lattice &g, const HouArgs &a, hipStream_t s) {
   const dim3 inst(1, g.ctm, 2 * g.ns);
   if (a.planes <= 0) {
     const size_t lds = hou_whole_lds(g);
-    allow_lds(reinterpret_cast<const void *>(k_hou_whole), lds);
-    hipLaunchKernelGGL(k_hou_whole, inst, dim3(1024), lds, s, g, a);
+    if (g.L <= 32) {
+      allow_lds(reinterpret_cast<const void *>(k_hou_whole<uint32_t>), lds);
+      hipLaunchKernelGGL(k_hou_whole<uint32_t>, inst, dim3(1024), lds, s, g, a);
+    } else {
+      allow_lds(reinterpret_cast<const void *>(k_hou_whole<uint64_t>), lds);
+      hipLaunchKernelGGL(k_hou_whole<uint64_t>, inst, dim3(1024), lds, s, g, a);
+    }
     return hipGetLastError();
   }
   const int slabs = (g.Lz + a.planes - 1) / a.planes;
