@@ -1,4 +1,9 @@
 // engine.hip -- the C ABI of include/eatube.h over the gfx950 kernels.
+//
+// An engine is one tube: its device state, a one-entry device tube table and
+// the block maps of that table.  A batch is a table of several engines'
+// tubes whose step (sweeps, PT, Houdayer, measures) is captured once in a
+// HIP graph and replayed; the kernels are the same.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,6 +40,12 @@ int fail(int code, const char *fmt, ...) {
       return fail(EA_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
+#define RC_TRY(call)        \
+  do {                      \
+    const int rc_ = (call); \
+    if (rc_) return rc_;    \
+  } while (0)
+
 template <typename T>
 int dalloc(T **p, size_t count) {
   *p = nullptr;
@@ -46,36 +57,35 @@ int dalloc(T **p, size_t count) {
   return EA_OK;
 }
 
+// Shape of one step of ea_run (main.cu:149-228): the loops whose launches a
+// step holds.
+struct StepShape {
+  int mesfr = 0, houfr = 0, ptfr = 0;
+  bool cluster = false;   // some tube has cluster moves
+  bool unfused = false;   // some tube measures with a separate energy pass
+  bool operator==(const StepShape &o) const {
+    return mesfr == o.mesfr && houfr == o.houfr && ptfr == o.ptfr && cluster == o.cluster &&
+           unfused == o.unfused;
+  }
+};
+
 }  // namespace
 
 struct ea_engine {
   int device = 0;
   hipStream_t stream = nullptr;
-  Lattice g{};
+  Tube t{};             // host copy of the descriptor
+  Tube *dt = nullptr;   // device copy
+  DevPlan plan;
   uint32_t job = 0;
   std::vector<double> betas;
-  uint32_t keys[EA_MAX_REPLICAS] = {};
-
-  uint64_t *spin = nullptr;   // [ns][2][nt][n][nr]
-  uint64_t *J = nullptr;      // [ns][2][n][6]
-  uint8_t *wctb = nullptr;    // [ns][nr][nt]
-  uint8_t *lut = nullptr;     // [nt][EA_LUT_BYTES]
-  int32_t *ener = nullptr;    // [ns][nr][nt]
-  int32_t *part = nullptr;    // partial energies (see energy_partials)
+  size_t ctr_cap = 0, pairs_cap = 0;
+  int hist_rows = 0;
+  int keep_on = 0;
+  uint32_t *keep_buf = nullptr;
   // energy bookkeeping: the last sweep left partials by beta that no kernel
   // has summed yet (part_pending), or ener holds the current energies
   bool part_pending = false, ener_valid = false;
-  ea_hq_state *pt = nullptr;  // [ns][nr]
-  double *acc = nullptr, *att = nullptr, *raw = nullptr;
-  double *nd = nullptr, *tab = nullptr;
-  int32_t *toff = nullptr, *tlen = nullptr;
-  uint32_t *hlab = nullptr, *hcb = nullptr;  // tiled Houdayer labels / coins
-  int planes = 0;
-  uint32_t *keep = nullptr;
-  int keep_on = 0;
-  double *esum = nullptr;
-  uint8_t *hist = nullptr;
-  int hist_rows = 0;
   // kernel timing: pending (kind, start, stop) event triples
   bool timing = false;
   struct Mark {
@@ -86,6 +96,8 @@ struct ea_engine {
   std::vector<hipEvent_t> spare;
   double ms[EA_K_KINDS] = {};
   int64_t launches[EA_K_KINDS] = {};
+
+  const Lattice &g() const { return t.g; }
 
   hipEvent_t event() {
     if (!spare.empty()) {
@@ -104,10 +116,33 @@ struct ea_engine {
       (void)hipEventDestroy(m.b);
     }
     for (auto ev : spare) (void)hipEventDestroy(ev);
-    void *ptrs[] = {spin, J,   wctb, lut,  ener, part, pt,   acc,  att, raw,
-                    nd,   tab, toff, tlen, hlab, hcb,  keep, esum, hist};
-    for (void *p : ptrs)
-      if (p) (void)hipFree(p);
+    free_plan(&plan);
+    const void *ptrs[] = {t.spin, t.J,    t.wctb, t.lut,  t.ener, t.part,     t.pt,
+                          t.acc,  t.att,  t.raw,  t.nd,   t.tab,  t.toff,     t.tlen,
+                          t.esum, t.hist, t.hlab, t.hcb,  t.step, t.ctr,      t.pairs,
+                          keep_buf, dt};
+    for (const void *p : ptrs)
+      if (p) (void)hipFree(const_cast<void *>(p));
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct ea_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<ea_engine *> eng;
+  Tube *dt = nullptr;
+  DevPlan plan;
+  std::vector<int> plan_cluster;  // cluster flags the plan was built for
+  StepShape shape;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~ea_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    free_plan(&plan);
+    if (dt) (void)hipFree(dt);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -147,6 +182,71 @@ static int check_config(const ea_config *c) {
   return EA_OK;
 }
 
+// Waits for the engine's stream and copies the host descriptor to the device.
+static int upload_tube(ea_engine *e) {
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HIP_TRY(hipMemcpy(e->dt, &e->t, sizeof(Tube), hipMemcpyHostToDevice));
+  return EA_OK;
+}
+
+// Sets up the calls in flight: nsteps steps of `halves` half-sweeps and
+// `calls` Houdayer calls, step counter at 0.
+static int begin_calls(ea_engine *e, int nsteps, int halves, int calls, int it0, int frec) {
+  e->t.nsteps = nsteps;
+  e->t.halves = halves;
+  e->t.calls = calls;
+  e->t.it0 = it0;
+  e->t.frec = std::max(frec, 1);
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  const int zero = 0;
+  HIP_TRY(hipMemcpy(e->t.step, &zero, sizeof zero, hipMemcpyHostToDevice));
+  return upload_tube(e);
+}
+
+// Host inputs of the calls in flight: counter blocks and replica pairs.
+static int stage_inputs(ea_engine *e, const ea_counter *ctr, size_t nctr, const int32_t *pairs,
+                        size_t npairs) {
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (nctr > e->ctr_cap) {
+    if (e->t.ctr) HIP_TRY(hipFree(const_cast<ea_counter *>(e->t.ctr)));
+    e->t.ctr = nullptr;
+    e->ctr_cap = 0;
+    ea_counter *p = nullptr;
+    RC_TRY(dalloc(&p, nctr));
+    e->t.ctr = p;
+    e->ctr_cap = nctr;
+  }
+  if (npairs > e->pairs_cap) {
+    if (e->t.pairs) HIP_TRY(hipFree(const_cast<int32_t *>(e->t.pairs)));
+    e->t.pairs = nullptr;
+    e->pairs_cap = 0;
+    int32_t *p = nullptr;
+    RC_TRY(dalloc(&p, npairs));
+    e->t.pairs = p;
+    e->pairs_cap = npairs;
+  }
+  if (nctr)
+    HIP_TRY(hipMemcpy(const_cast<ea_counter *>(e->t.ctr), ctr, sizeof(ea_counter) * nctr,
+                      hipMemcpyHostToDevice));
+  if (npairs)
+    HIP_TRY(hipMemcpy(const_cast<int32_t *>(e->t.pairs), pairs, sizeof(int32_t) * npairs,
+                      hipMemcpyHostToDevice));
+  return EA_OK;
+}
+
+static int ensure_hist(ea_engine *e, int rows) {
+  const Lattice &g = e->g();
+  if (rows <= e->hist_rows) return EA_OK;
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (e->t.hist) HIP_TRY(hipFree(e->t.hist));
+  e->t.hist = nullptr;
+  e->hist_rows = 0;
+  RC_TRY(dalloc(&e->t.hist, (size_t)rows * g.ns * g.nr * g.nt));
+  HIP_TRY(hipMemset(e->t.hist, 0, (size_t)rows * g.ns * g.nr * g.nt));
+  e->hist_rows = rows;
+  return EA_OK;
+}
+
 extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   if (!out) return fail(EA_E_ARG, "out is NULL");
   *out = nullptr;
@@ -165,7 +265,8 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
 
   auto e = std::make_unique<ea_engine>();
   e->device = dev;
-  Lattice &g = e->g;
+  Tube &t = e->t;
+  Lattice &g = t.g;
   g.L = cfg->L;
   g.Lz = cfg->Lz;
   g.ML = cfg->L / 4;
@@ -183,46 +284,64 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   g.ctm = cfg->clone_threshold;
   g.half = (int64_t)g.n * g.nr * g.nt;
   set_divisors(g);
+  t.nbx = sweep_blocks_x(g);
+  t.frec = 1;
+  t.cluster = g.ctm > 0 && g.nr >= 4;
   e->job = cfg->job_id;
   e->betas.assign(cfg->betas, cfg->betas + cfg->nt);
 
   HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   const size_t chains = (size_t)g.ns * g.nr;
-  if ((rc = dalloc(&e->spin, (size_t)g.ns * 2 * g.half)) ||
-      (rc = dalloc(&e->J, (size_t)g.ns * 12 * g.n)) ||
-      (rc = dalloc(&e->wctb, chains * g.nt)) ||
-      (rc = dalloc(&e->lut, (size_t)g.nt * EA_LUT_BYTES)) ||
-      (rc = dalloc(&e->ener, chains * g.nt)) || (rc = dalloc(&e->pt, chains)) ||
-      (rc = dalloc(&e->part, energy_partials(g))) ||
-      (rc = dalloc(&e->acc, chains * (g.nt - 1))) ||
-      (rc = dalloc(&e->att, chains * (g.nt - 1))) ||
-      (rc = dalloc(&e->raw, chains * (g.nt - 1))) ||
-      (rc = dalloc(&e->esum, chains * g.nt)))
+  uint64_t *J = nullptr;
+  uint8_t *lut = nullptr;
+  if ((rc = dalloc(&t.spin, (size_t)g.ns * 2 * g.half)) ||
+      (rc = dalloc(&J, (size_t)g.ns * 12 * g.n)) || (rc = dalloc(&t.wctb, chains * g.nt)) ||
+      (rc = dalloc(&lut, (size_t)g.nt * EA_LUT_BYTES)) ||
+      (rc = dalloc(&t.ener, chains * g.nt)) || (rc = dalloc(&t.pt, chains)) ||
+      (rc = dalloc(&t.part, energy_partials(g))) ||
+      (rc = dalloc(&t.acc, chains * (g.nt - 1))) ||
+      (rc = dalloc(&t.att, chains * (g.nt - 1))) ||
+      (rc = dalloc(&t.raw, chains * (g.nt - 1))) || (rc = dalloc(&t.esum, chains * g.nt)) ||
+      (rc = dalloc(&t.step, 1)) || (rc = dalloc(&e->dt, 1))) {
+    t.J = J;
+    t.lut = lut;
     return rc;
-  HIP_TRY(hipMemset(e->spin, 0, sizeof(uint64_t) * g.ns * 2 * g.half));
-  HIP_TRY(hipMemset(e->J, 0, sizeof(uint64_t) * g.ns * 12 * g.n));
-  HIP_TRY(hipMemcpy(e->lut, cfg->lut, (size_t)g.nt * EA_LUT_BYTES, hipMemcpyHostToDevice));
+  }
+  t.J = J;
+  t.lut = lut;
+  HIP_TRY(hipMemset(t.spin, 0, sizeof(uint64_t) * g.ns * 2 * g.half));
+  HIP_TRY(hipMemset(J, 0, sizeof(uint64_t) * g.ns * 12 * g.n));
+  HIP_TRY(hipMemcpy(lut, cfg->lut, (size_t)g.nt * EA_LUT_BYTES, hipMemcpyHostToDevice));
   std::vector<uint8_t> ident(chains * g.nt);
   for (size_t c = 0; c < chains; ++c)
     for (int b = 0; b < g.nt; ++b) ident[c * g.nt + b] = (uint8_t)b;
-  HIP_TRY(hipMemcpy(e->wctb, ident.data(), ident.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemset(e->ener, 0, sizeof(int32_t) * chains * g.nt));
-  HIP_TRY(hipMemset(e->pt, 0, sizeof(ea_hq_state) * chains));
-  HIP_TRY(hipMemset(e->acc, 0, sizeof(double) * chains * (g.nt - 1)));
-  HIP_TRY(hipMemset(e->att, 0, sizeof(double) * chains * (g.nt - 1)));
-  HIP_TRY(hipMemset(e->raw, 0, sizeof(double) * chains * (g.nt - 1)));
-  HIP_TRY(hipMemset(e->esum, 0, sizeof(double) * chains * g.nt));
+  HIP_TRY(hipMemcpy(t.wctb, ident.data(), ident.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(t.ener, 0, sizeof(int32_t) * chains * g.nt));
+  HIP_TRY(hipMemset(t.pt, 0, sizeof(ea_hq_state) * chains));
+  HIP_TRY(hipMemset(t.acc, 0, sizeof(double) * chains * (g.nt - 1)));
+  HIP_TRY(hipMemset(t.att, 0, sizeof(double) * chains * (g.nt - 1)));
+  HIP_TRY(hipMemset(t.raw, 0, sizeof(double) * chains * (g.nt - 1)));
+  HIP_TRY(hipMemset(t.esum, 0, sizeof(double) * chains * g.nt));
+  HIP_TRY(hipMemset(t.step, 0, sizeof(int)));
 
   // exact exp(pt) tables for the device parallel tempering
-  const ExpTables t = make_exp_tables(cfg->betas, g.nt, g.V);
-  if ((rc = dalloc(&e->nd, t.nd.size())) || (rc = dalloc(&e->toff, t.off.size())) ||
-      (rc = dalloc(&e->tlen, t.len.size())) || (rc = dalloc(&e->tab, std::max<size_t>(t.val.size(), 1))))
-    return rc;
-  HIP_TRY(hipMemcpy(e->nd, t.nd.data(), sizeof(double) * t.nd.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(e->toff, t.off.data(), sizeof(int32_t) * t.off.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(e->tlen, t.len.data(), sizeof(int32_t) * t.len.size(), hipMemcpyHostToDevice));
-  if (!t.val.empty())
-    HIP_TRY(hipMemcpy(e->tab, t.val.data(), sizeof(double) * t.val.size(), hipMemcpyHostToDevice));
+  const ExpTables x = make_exp_tables(cfg->betas, g.nt, g.V);
+  double *nd = nullptr, *tab = nullptr;
+  int32_t *toff = nullptr, *tlen = nullptr;
+  rc = dalloc(&nd, x.nd.size());
+  if (!rc) rc = dalloc(&toff, x.off.size());
+  if (!rc) rc = dalloc(&tlen, x.len.size());
+  if (!rc) rc = dalloc(&tab, std::max<size_t>(x.val.size(), 1));
+  t.nd = nd;
+  t.tab = tab;
+  t.toff = toff;
+  t.tlen = tlen;
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(nd, x.nd.data(), sizeof(double) * x.nd.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(toff, x.off.data(), sizeof(int32_t) * x.off.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(tlen, x.len.data(), sizeof(int32_t) * x.len.size(), hipMemcpyHostToDevice));
+  if (!x.val.empty())
+    HIP_TRY(hipMemcpy(tab, x.val.data(), sizeof(double) * x.val.size(), hipMemcpyHostToDevice));
 
   // Houdayer: a tube whose labels fit one workgroup's LDS is labelled there,
   // larger tubes slab by slab with a merge in HBM
@@ -236,12 +355,14 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
     if (hou_whole_lds(g) > (size_t)lds_max) {
       int planes = std::max(1, std::min(g.Lz, kHouSlabSites / (g.L * g.L)));
       while (planes > 1 && hou_slab_lds(g, planes) > (size_t)lds_max) --planes;
-      e->planes = planes;
+      t.planes = planes;
       const size_t inst = (size_t)2 * g.ns * g.ctm;
-      if ((rc = dalloc(&e->hlab, inst * g.V)) || (rc = dalloc(&e->hcb, inst * (g.V / 32))))
+      if ((rc = dalloc(&t.hlab, inst * g.V)) || (rc = dalloc(&t.hcb, inst * (g.V / 32))))
         return rc;
     }
   }
+  HIP_TRY(upload_plan(make_plan({t}), &e->plan));
+  RC_TRY(upload_tube(e.get()));
   *out = e.release();
   return EA_OK;
 }
@@ -261,17 +382,17 @@ extern "C" int ea_sync(ea_engine *e) {
 
 extern "C" int ea_set_keys(ea_engine *e, const uint32_t *keys) {
   if (!e || !keys) return fail(EA_E_ARG, "NULL argument");
-  std::memcpy(e->keys, keys, sizeof(uint32_t) * e->g.nr);
-  return EA_OK;
+  std::memcpy(e->t.keys, keys, sizeof(uint32_t) * e->g().nr);
+  return upload_tube(e);
 }
 
 extern "C" int ea_upload_sample(ea_engine *e, int s, const uint64_t *even,
                                 const uint64_t *odd, const uint64_t *J) {
   if (!e || !even || !odd || !J) return fail(EA_E_ARG, "NULL argument");
-  const Lattice &g = e->g;
+  const Lattice &g = e->g();
   if (s < 0 || s >= g.ns) return fail(EA_E_ARG, "sample %d out of range", s);
   HIP_TRY(hipStreamSynchronize(e->stream));
-  uint64_t *base = e->spin + (size_t)s * 2 * g.half;
+  uint64_t *base = e->t.spin + (size_t)s * 2 * g.half;
   HIP_TRY(hipMemcpy(base, even, sizeof(uint64_t) * g.half, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(base + g.half, odd, sizeof(uint64_t) * g.half, hipMemcpyHostToDevice));
   // [parity][direction][h] -> [parity][h][direction]
@@ -280,18 +401,18 @@ extern "C" int ea_upload_sample(ea_engine *e, int s, const uint64_t *even,
     for (int d = 0; d < 6; ++d)
       for (int h = 0; h < g.n; ++h)
         t[((size_t)p * g.n + h) * 6 + d] = J[((size_t)p * 6 + d) * g.n + h];
-  HIP_TRY(hipMemcpy(e->J + (size_t)s * 12 * g.n, t.data(), sizeof(uint64_t) * t.size(),
-                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(const_cast<uint64_t *>(e->t.J) + (size_t)s * 12 * g.n, t.data(),
+                    sizeof(uint64_t) * t.size(), hipMemcpyHostToDevice));
   e->part_pending = e->ener_valid = false;
   return EA_OK;
 }
 
 extern "C" int ea_download_spins(ea_engine *e, int s, uint64_t *even, uint64_t *odd) {
   if (!e || !even || !odd) return fail(EA_E_ARG, "NULL argument");
-  const Lattice &g = e->g;
+  const Lattice &g = e->g();
   if (s < 0 || s >= g.ns) return fail(EA_E_ARG, "sample %d out of range", s);
   HIP_TRY(hipStreamSynchronize(e->stream));
-  const uint64_t *base = e->spin + (size_t)s * 2 * g.half;
+  const uint64_t *base = e->t.spin + (size_t)s * 2 * g.half;
   HIP_TRY(hipMemcpy(even, base, sizeof(uint64_t) * g.half, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(odd, base + g.half, sizeof(uint64_t) * g.half, hipMemcpyDeviceToHost));
   return EA_OK;
@@ -299,7 +420,7 @@ extern "C" int ea_download_spins(ea_engine *e, int s, uint64_t *even, uint64_t *
 
 extern "C" int ea_upload_perms(ea_engine *e, const uint8_t *wctb) {
   if (!e || !wctb) return fail(EA_E_ARG, "NULL argument");
-  const Lattice &g = e->g;
+  const Lattice &g = e->g();
   for (int c = 0; c < g.ns * g.nr; ++c) {
     unsigned seen[EA_MAX_BETAS + 1] = {};
     for (int b = 0; b < g.nt; ++b) {
@@ -309,40 +430,40 @@ extern "C" int ea_upload_perms(ea_engine *e, const uint8_t *wctb) {
     }
   }
   HIP_TRY(hipStreamSynchronize(e->stream));
-  HIP_TRY(hipMemcpy(e->wctb, wctb, (size_t)g.ns * g.nr * g.nt, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(e->t.wctb, wctb, (size_t)g.ns * g.nr * g.nt, hipMemcpyHostToDevice));
   e->part_pending = e->ener_valid = false;  // partials by beta map through the old perms
   return EA_OK;
 }
 
 extern "C" int ea_download_perms(ea_engine *e, uint8_t *wctb) {
   if (!e || !wctb) return fail(EA_E_ARG, "NULL argument");
+  const Lattice &g = e->g();
   HIP_TRY(hipStreamSynchronize(e->stream));
-  HIP_TRY(hipMemcpy(wctb, e->wctb, (size_t)e->g.ns * e->g.nr * e->g.nt,
-                    hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(wctb, e->t.wctb, (size_t)g.ns * g.nr * g.nt, hipMemcpyDeviceToHost));
   return EA_OK;
 }
 
 extern "C" int ea_upload_pt_state(ea_engine *e, const ea_hq_state *st, const double *acc,
                                   const double *att, const double *raw) {
   if (!e) return fail(EA_E_ARG, "engine is NULL");
-  const size_t chains = (size_t)e->g.ns * e->g.nr, np = chains * (e->g.nt - 1);
+  const size_t chains = (size_t)e->g().ns * e->g().nr, np = chains * (e->g().nt - 1);
   HIP_TRY(hipStreamSynchronize(e->stream));
-  if (st) HIP_TRY(hipMemcpy(e->pt, st, sizeof(ea_hq_state) * chains, hipMemcpyHostToDevice));
-  if (acc) HIP_TRY(hipMemcpy(e->acc, acc, sizeof(double) * np, hipMemcpyHostToDevice));
-  if (att) HIP_TRY(hipMemcpy(e->att, att, sizeof(double) * np, hipMemcpyHostToDevice));
-  if (raw) HIP_TRY(hipMemcpy(e->raw, raw, sizeof(double) * np, hipMemcpyHostToDevice));
+  if (st) HIP_TRY(hipMemcpy(e->t.pt, st, sizeof(ea_hq_state) * chains, hipMemcpyHostToDevice));
+  if (acc) HIP_TRY(hipMemcpy(e->t.acc, acc, sizeof(double) * np, hipMemcpyHostToDevice));
+  if (att) HIP_TRY(hipMemcpy(e->t.att, att, sizeof(double) * np, hipMemcpyHostToDevice));
+  if (raw) HIP_TRY(hipMemcpy(e->t.raw, raw, sizeof(double) * np, hipMemcpyHostToDevice));
   return EA_OK;
 }
 
 extern "C" int ea_download_pt_state(ea_engine *e, ea_hq_state *st, double *acc,
                                     double *att, double *raw) {
   if (!e) return fail(EA_E_ARG, "engine is NULL");
-  const size_t chains = (size_t)e->g.ns * e->g.nr, np = chains * (e->g.nt - 1);
+  const size_t chains = (size_t)e->g().ns * e->g().nr, np = chains * (e->g().nt - 1);
   HIP_TRY(hipStreamSynchronize(e->stream));
-  if (st) HIP_TRY(hipMemcpy(st, e->pt, sizeof(ea_hq_state) * chains, hipMemcpyDeviceToHost));
-  if (acc) HIP_TRY(hipMemcpy(acc, e->acc, sizeof(double) * np, hipMemcpyDeviceToHost));
-  if (att) HIP_TRY(hipMemcpy(att, e->att, sizeof(double) * np, hipMemcpyDeviceToHost));
-  if (raw) HIP_TRY(hipMemcpy(raw, e->raw, sizeof(double) * np, hipMemcpyDeviceToHost));
+  if (st) HIP_TRY(hipMemcpy(st, e->t.pt, sizeof(ea_hq_state) * chains, hipMemcpyDeviceToHost));
+  if (acc) HIP_TRY(hipMemcpy(acc, e->t.acc, sizeof(double) * np, hipMemcpyDeviceToHost));
+  if (att) HIP_TRY(hipMemcpy(att, e->t.att, sizeof(double) * np, hipMemcpyDeviceToHost));
+  if (raw) HIP_TRY(hipMemcpy(raw, e->t.raw, sizeof(double) * np, hipMemcpyDeviceToHost));
   return EA_OK;
 }
 
@@ -354,7 +475,7 @@ struct Timed {
   int kind;
   hipEvent_t a = nullptr;
   Timed(ea_engine *e_, int kind_) : e(e_), kind(kind_) {
-    if (e->timing) {
+    if (e && e->timing) {
       a = e->event();
       (void)hipEventRecord(a, e->stream);
     }
@@ -368,133 +489,99 @@ struct Timed {
   }
 };
 
-static int enqueue_sweeps(ea_engine *e, int nsweeps, const ea_counter *blocks,
-                          bool energy) {
-  SweepArgs a{};
-  a.spin = e->spin;
-  a.J = e->J;
-  a.wctb = e->wctb;
-  a.lut = e->lut;
-  a.part = e->part;
-  std::memcpy(a.keys, e->keys, sizeof a.keys);
-  const int nr = e->g.nr;
-  for (int sw = 0; sw < nsweeps; ++sw) {
-    const bool last = energy && sw == nsweeps - 1;
-    for (int p = 0; p < 2; ++p) {
-      a.parity = p;
-      a.energy = last && p == 1;
-      std::memcpy(a.ctr, blocks + ((size_t)sw * 2 + p) * nr, sizeof(ea_counter) * nr);
-      Timed t(e, EA_K_SWEEP);
-      HIP_TRY(launch_sweep(e->g, a, e->stream));
+// Where the launches of one step go: a tube table, its block maps and a
+// stream; `timer` is the engine whose timing brackets them (NULL in a batch).
+struct Target {
+  const Tube *dt;
+  const DevPlan *plan;
+  int ntubes;
+  hipStream_t stream;
+  ea_engine *timer;
+};
+
+// One step of main()'s `it` loop (main.cu:149-228) for every tube of the
+// table: mesfr x { houfr x { ptfr sweeps, PT }, Houdayer }, then the energy
+// measure of tubes that do not measure inside PT, and the step counter.
+// Every PT follows a sweep, so PT takes its energies from the sweep's
+// partials.
+static int enqueue_step(const Target &x, const StepShape &sh) {
+  int half = 0;
+  for (int im = 0; im < sh.mesfr; ++im) {
+    const bool last_m = im == sh.mesfr - 1;
+    for (int ih = 0; ih < sh.houfr; ++ih) {
+      for (int sw = 0; sw < sh.ptfr; ++sw)
+        for (int p = 0; p < 2; ++p) {
+          Phase ph{half++, sw == sh.ptfr - 1 && p == 1, 0, 0};
+          Timed tm(x.timer, EA_K_SWEEP);
+          HIP_TRY(launch_sweep(x.dt, *x.plan, ph, x.stream));
+        }
+      Phase ph{0, 1, 0, last_m && ih == sh.houfr - 1};
+      Timed tm(x.timer, EA_K_PT);
+      HIP_TRY(launch_pt(x.dt, *x.plan, ph, x.stream));
+    }
+    if (sh.cluster) {
+      Phase ph{0, 0, im, last_m};
+      Timed tm(x.timer, EA_K_HOUDAYER);
+      HIP_TRY(launch_houdayer(x.dt, *x.plan, ph, x.stream));
     }
   }
-  if (nsweeps > 0) {
-    e->part_pending = energy;
-    e->ener_valid = false;
+  if (sh.unfused) {
+    {
+      Timed tm(x.timer, EA_K_ENERGY);
+      HIP_TRY(launch_energy(x.dt, *x.plan, true, x.stream));
+    }
+    Timed tm(x.timer, EA_K_MEASURE);
+    HIP_TRY(launch_measure(x.dt, *x.plan, true, x.stream));
   }
+  HIP_TRY(launch_step_end(x.dt, x.ntubes, x.stream));
   return EA_OK;
 }
 
+static Target engine_target(ea_engine *e) { return {e->dt, &e->plan, 1, e->stream, e}; }
+
 // Makes ener hold the current energies by clone: sums the last sweep's
-// partials, or recomputes them from the spins.
+// partials, or recomputes them from the spins.  Needs a call in flight.
 static int ensure_energy(ea_engine *e) {
   if (e->part_pending) {
     Timed t(e, EA_K_ENERGY);
-    HIP_TRY(launch_resolve(e->g, e->part, e->wctb, e->ener, e->stream));
+    HIP_TRY(launch_resolve(e->dt, e->plan, e->stream));
   } else if (!e->ener_valid) {
     Timed t(e, EA_K_ENERGY);
-    HIP_TRY(launch_energy(e->g, e->spin, e->J, e->ener, e->part, e->stream));
+    HIP_TRY(launch_energy(e->dt, e->plan, false, e->stream));
   }
   e->part_pending = false;
   e->ener_valid = true;
-  return EA_OK;
-}
-
-// measure: add the per-beta energies after the swaps to esum and, for
-// row >= 0, write that history row.
-static int enqueue_pt(ea_engine *e, bool measure = false, int row = -1) {
-  PtArgs a{};
-  if (e->part_pending) {
-    a.part = e->part;
-    a.nbx = sweep_blocks_x(e->g);
-  } else {
-    const int rc = ensure_energy(e);
-    if (rc) return rc;
-  }
-  a.st = e->pt;
-  a.ener = e->ener;
-  a.wctb = e->wctb;
-  a.acc = e->acc;
-  a.att = e->att;
-  a.raw = e->raw;
-  a.nd = e->nd;
-  a.tab = e->tab;
-  a.off = e->toff;
-  a.len = e->tlen;
-  a.esum = e->esum;
-  a.hist = e->hist;
-  a.measure = measure;
-  a.row = row;
-  Timed t(e, EA_K_PT);
-  HIP_TRY(launch_pt(e->g, a, e->stream));
-  e->part_pending = false;
-  e->ener_valid = true;
-  return EA_OK;
-}
-
-// measure (whole-tube path only): correct the per-beta sums a measuring PT
-// added just before for the configurations the move changes.
-static int enqueue_houdayer(ea_engine *e, const int32_t pairs[4], uint64_t seed,
-                            uint64_t offset, bool measure = false) {
-  const int half = e->g.nr / 2;
-  for (int k = 0; k < 4; ++k) {
-    const int lo = (k / 2) * half;
-    if (pairs[k] < lo || pairs[k] >= lo + half)
-      return fail(EA_E_ARG, "replica pair entry %d = %d outside group %d", k, pairs[k], k / 2);
-  }
-  if (pairs[0] == pairs[1] || pairs[2] == pairs[3])
-    return fail(EA_E_ARG, "a Houdayer pair needs two distinct replicas");
-  if (e->planes == 0) {  // the whole-tube kernel updates the energies it moves
-    const int rc = ensure_energy(e);
-    if (rc) return rc;
-  }
-  HouArgs a{};
-  a.spin = e->spin;
-  a.J = e->J;
-  a.ener = e->ener;
-  a.wctb = e->wctb;
-  std::memcpy(a.pairs, pairs, sizeof a.pairs);
-  a.seed = seed;
-  a.offset0 = offset;
-  a.lab = e->hlab;
-  a.cbits = e->hcb;
-  a.labels_out = e->keep_on ? e->keep : nullptr;
-  a.planes = e->planes;
-  a.esum = e->esum;
-  a.measure = measure && e->planes == 0;
-  Timed t(e, EA_K_HOUDAYER);
-  HIP_TRY(launch_houdayer(e->g, a, e->stream));
-  if (e->planes > 0) e->part_pending = e->ener_valid = false;
   return EA_OK;
 }
 
 extern "C" int ea_sweep(ea_engine *e, int nsweeps, const ea_counter *blocks) {
   if (!e || (nsweeps > 0 && !blocks)) return fail(EA_E_ARG, "NULL argument");
   if (nsweeps < 0) return fail(EA_E_ARG, "nsweeps < 0");
-  return enqueue_sweeps(e, nsweeps, blocks, true);
+  if (nsweeps == 0) return EA_OK;
+  RC_TRY(stage_inputs(e, blocks, (size_t)nsweeps * 2 * e->g().nr, nullptr, 0));
+  RC_TRY(begin_calls(e, 1, 2 * nsweeps, 0, 0, 1));
+  for (int half = 0; half < 2 * nsweeps; ++half) {
+    Phase ph{half, half == 2 * nsweeps - 1, 0, 0};
+    Timed t(e, EA_K_SWEEP);
+    HIP_TRY(launch_sweep(e->dt, e->plan, ph, e->stream));
+  }
+  e->part_pending = true;
+  e->ener_valid = false;
+  return EA_OK;
 }
 
 extern "C" int ea_energy(ea_engine *e, int32_t *out) {
   if (!e) return fail(EA_E_ARG, "engine is NULL");
-  const Lattice &g = e->g;
+  const Lattice &g = e->g();
+  RC_TRY(begin_calls(e, 1, 0, 0, 0, 1));
   {
     Timed t(e, EA_K_ENERGY);
-    HIP_TRY(launch_energy(g, e->spin, e->J, e->ener, e->part, e->stream));
+    HIP_TRY(launch_energy(e->dt, e->plan, false, e->stream));
   }
   e->part_pending = false;
   e->ener_valid = true;
   if (out) {
-    HIP_TRY(hipMemcpyAsync(out, e->ener, sizeof(int32_t) * g.ns * g.nr * g.nt,
+    HIP_TRY(hipMemcpyAsync(out, e->t.ener, sizeof(int32_t) * g.ns * g.nr * g.nt,
                            hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
   }
@@ -503,114 +590,149 @@ extern "C" int ea_energy(ea_engine *e, int32_t *out) {
 
 extern "C" int ea_pt(ea_engine *e) {
   if (!e) return fail(EA_E_ARG, "engine is NULL");
-  return enqueue_pt(e);
+  RC_TRY(begin_calls(e, 1, 0, 0, 0, 1));
+  Phase ph{0, e->part_pending ? 1 : 0, 0, 0};
+  if (!e->part_pending) RC_TRY(ensure_energy(e));
+  {
+    Timed t(e, EA_K_PT);
+    HIP_TRY(launch_pt(e->dt, e->plan, ph, e->stream));
+  }
+  e->part_pending = false;
+  e->ener_valid = true;
+  return EA_OK;
+}
+
+static int check_pairs(const ea_engine *e, const int32_t *pairs) {
+  const int half = e->g().nr / 2;
+  for (int k = 0; k < 4; ++k) {
+    const int lo = (k / 2) * half;
+    if (pairs[k] < lo || pairs[k] >= lo + half)
+      return fail(EA_E_ARG, "replica pair entry %d = %d outside group %d", k, pairs[k], k / 2);
+  }
+  if (pairs[0] == pairs[1] || pairs[2] == pairs[3])
+    return fail(EA_E_ARG, "a Houdayer pair needs two distinct replicas");
+  return EA_OK;
+}
+
+static uint64_t coin_per_call(const Lattice &g) { return 2ull * g.V * g.nt * g.ns; }
+
+// The block maps follow the tube's cluster flag.
+static int set_cluster(ea_engine *e, int cluster) {
+  if (e->t.cluster == cluster) return EA_OK;
+  e->t.cluster = cluster;
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HIP_TRY(upload_plan(make_plan({e->t}), &e->plan));
+  return EA_OK;
 }
 
 extern "C" int ea_houdayer(ea_engine *e, const int32_t pairs[4], uint64_t coin_seed,
                            uint64_t *rng_offset) {
   if (!e || !pairs || !rng_offset) return fail(EA_E_ARG, "NULL argument");
-  if (e->g.nr < 4) return fail(EA_E_STATE, "the Houdayer move needs nr >= 4");
-  const int rc = enqueue_houdayer(e, pairs, coin_seed, *rng_offset);
-  if (rc) return rc;
-  *rng_offset += 2ull * e->g.V * e->g.nt * e->g.ns;
+  if (e->g().nr < 4 || e->g().ctm <= 0)
+    return fail(EA_E_STATE, "the Houdayer move needs nr >= 4 and clone_threshold > 0");
+  RC_TRY(check_pairs(e, pairs));
+  RC_TRY(set_cluster(e, 1));
+  RC_TRY(stage_inputs(e, nullptr, 0, pairs, 4));
+  e->t.coin_seed = coin_seed;
+  e->t.coin0 = *rng_offset;
+  e->t.coin_per_call = coin_per_call(e->g());
+  RC_TRY(begin_calls(e, 1, 0, 1, 0, 1));
+  if (e->t.planes == 0) RC_TRY(ensure_energy(e));  // the whole-tube kernel updates ener
+  {
+    Timed t(e, EA_K_HOUDAYER);
+    HIP_TRY(launch_houdayer(e->dt, e->plan, Phase{0, 0, 0, 0}, e->stream));
+  }
+  if (e->t.planes > 0) e->part_pending = e->ener_valid = false;
+  *rng_offset += e->t.coin_per_call;
   return EA_OK;
 }
 
 extern "C" int ea_keep_labels(ea_engine *e, int on) {
   if (!e) return fail(EA_E_ARG, "engine is NULL");
-  if (on && !e->keep) {
-    const size_t n = (size_t)2 * e->g.ns * e->g.ctm * e->g.V;
-    const int rc = dalloc(&e->keep, std::max<size_t>(n, 1));
-    if (rc) return rc;
+  if (on && !e->keep_buf) {
+    const size_t n = (size_t)2 * e->g().ns * e->g().ctm * e->g().V;
+    RC_TRY(dalloc(&e->keep_buf, std::max<size_t>(n, 1)));
   }
   e->keep_on = on != 0;
-  return EA_OK;
+  e->t.keep = e->keep_on ? e->keep_buf : nullptr;
+  return upload_tube(e);
 }
 
 extern "C" int ea_download_labels(ea_engine *e, uint32_t *labels) {
   if (!e || !labels) return fail(EA_E_ARG, "NULL argument");
-  if (!e->keep) return fail(EA_E_STATE, "labels are kept only after ea_keep_labels(e, 1)");
+  if (!e->keep_buf) return fail(EA_E_STATE, "labels are kept only after ea_keep_labels(e, 1)");
   HIP_TRY(hipStreamSynchronize(e->stream));
-  HIP_TRY(hipMemcpy(labels, e->keep, sizeof(uint32_t) * 2 * e->g.ns * e->g.ctm * e->g.V,
+  HIP_TRY(hipMemcpy(labels, e->keep_buf,
+                    sizeof(uint32_t) * 2 * e->g().ns * e->g().ctm * e->g().V,
                     hipMemcpyDeviceToHost));
   return EA_OK;
 }
 
+static int check_schedule(const ea_engine *e, const ea_schedule *sch) {
+  if (sch->n_it < 0 || sch->mesfr < 1 || sch->houfr < 1 || sch->ptfr < 1 ||
+      sch->frec_trw < 1 || sch->it0 < 0 || (sch->n_it > 0 && !sch->blocks))
+    return fail(EA_E_ARG, "invalid schedule");
+  if (sch->cluster) {
+    if (!sch->pairs || e->g().nr < 4 || e->g().ctm <= 0)
+      return fail(EA_E_ARG, "the Houdayer move needs pairs, nr >= 4 and clone_threshold > 0");
+    for (int k = 0; k < sch->n_it * sch->mesfr; ++k) RC_TRY(check_pairs(e, sch->pairs + 4 * k));
+  }
+  return EA_OK;
+}
+
+// Loads a schedule's inputs into the engine's calls in flight.
+static int load_schedule(ea_engine *e, const ea_schedule *sch) {
+  const Lattice &g = e->g();
+  const int halves = 2 * sch->ptfr * sch->houfr * sch->mesfr;
+  RC_TRY(ensure_hist(e, (sch->it0 + sch->n_it) / sch->frec_trw));
+  RC_TRY(stage_inputs(e, sch->blocks, (size_t)sch->n_it * halves * g.nr,
+                      sch->cluster ? sch->pairs : nullptr,
+                      sch->cluster ? (size_t)sch->n_it * sch->mesfr * 4 : 0));
+  RC_TRY(set_cluster(e, sch->cluster ? 1 : 0));
+  e->t.coin_seed = sch->coin_seed;
+  e->t.coin0 = sch->rng_offset;
+  e->t.coin_per_call = coin_per_call(g);
+  return begin_calls(e, sch->n_it, halves, sch->mesfr, sch->it0, sch->frec_trw);
+}
+
+static void finish_schedule(ea_engine *e, const ea_schedule *sch, uint64_t *rng_offset_out) {
+  if (sch->n_it > 0) {
+    e->part_pending = false;
+    e->ener_valid = true;
+  }
+  if (rng_offset_out)
+    *rng_offset_out = sch->rng_offset + (sch->cluster ? (uint64_t)sch->n_it * sch->mesfr *
+                                                            e->t.coin_per_call
+                                                      : 0);
+}
+
 extern "C" int ea_run(ea_engine *e, const ea_schedule *sch, uint64_t *rng_offset_out) {
   if (!e || !sch) return fail(EA_E_ARG, "NULL argument");
-  const Lattice &g = e->g;
-  if (sch->n_it < 0 || sch->mesfr < 1 || sch->houfr < 1 || sch->ptfr < 1 ||
-      sch->frec_trw < 1 || sch->it0 < 0 || !sch->blocks)
-    return fail(EA_E_ARG, "invalid schedule");
-  if (sch->cluster && (!sch->pairs || g.nr < 4))
-    return fail(EA_E_ARG, "the Houdayer move needs pairs and nr >= 4");
-  const int rows = (sch->it0 + sch->n_it) / sch->frec_trw;
-  if (rows > e->hist_rows) {
-    if (e->hist) HIP_TRY(hipFree(e->hist));
-    e->hist = nullptr;
-    e->hist_rows = 0;
-    const int rc = dalloc(&e->hist, (size_t)rows * g.ns * g.nr * g.nt);
-    if (rc) return rc;
-    HIP_TRY(hipMemset(e->hist, 0, (size_t)rows * g.ns * g.nr * g.nt));
-    e->hist_rows = rows;
-  }
-  uint64_t offset = sch->rng_offset;
-  const uint64_t per_call = 2ull * g.V * g.nt * g.ns;
-  const size_t blocks_per_sweep = 2 * (size_t)g.nr;
-  size_t bi = 0;
-  // The step's energy measure (main.cu:208-228) rides on its last PT, which
-  // adds the energies after the swaps; the whole-tube cluster move then
-  // corrects the sums for what it changed.  The slab path recomputes the
-  // energies and measures separately.
-  const bool fused = !sch->cluster || e->planes == 0;
-  for (int it = 0; it < sch->n_it; ++it) {
-    const int itb = sch->it0 + it;
-    const int row = (itb % sch->frec_trw == sch->frec_trw - 1) ? itb / sch->frec_trw : -1;
-    for (int im = 0; im < sch->mesfr; ++im) {
-      const bool last_m = im == sch->mesfr - 1;
-      for (int ih = 0; ih < sch->houfr; ++ih) {
-        int rc = enqueue_sweeps(e, sch->ptfr, sch->blocks + bi * blocks_per_sweep, true);
-        if (rc) return rc;
-        bi += sch->ptfr;
-        const bool measure = fused && last_m && ih == sch->houfr - 1;
-        if ((rc = enqueue_pt(e, measure, measure ? row : -1))) return rc;
-      }
-      if (sch->cluster) {
-        const int rc = enqueue_houdayer(
-            e, sch->pairs + ((size_t)it * sch->mesfr + im) * 4, sch->coin_seed, offset,
-            fused && last_m);
-        if (rc) return rc;
-        offset += per_call;
-      }
-    }
-    if (!fused) {
-      int rc = ensure_energy(e);
-      if (rc) return rc;
-      MeasureArgs m{};
-      m.ener = e->ener;
-      m.wctb = e->wctb;
-      m.esum = e->esum;
-      m.hist = e->hist;
-      m.row = row;
-      Timed t(e, EA_K_MEASURE);
-      HIP_TRY(launch_measure(g, m, e->stream));
-    }
-  }
-  if (rng_offset_out) *rng_offset_out = offset;
+  RC_TRY(check_schedule(e, sch));
+  RC_TRY(load_schedule(e, sch));
+  StepShape sh;
+  sh.mesfr = sch->mesfr;
+  sh.houfr = sch->houfr;
+  sh.ptfr = sch->ptfr;
+  sh.cluster = sch->cluster != 0;
+  sh.unfused = !measured_in_pt(e->t);
+  const Target x = engine_target(e);
+  for (int it = 0; it < sch->n_it; ++it) RC_TRY(enqueue_step(x, sh));
+  finish_schedule(e, sch, rng_offset_out);
   return EA_OK;
 }
 
 extern "C" int ea_read_measures(ea_engine *e, double *esum, uint8_t *history, int max_rows) {
   if (!e) return fail(EA_E_ARG, "engine is NULL");
-  const Lattice &g = e->g;
+  const Lattice &g = e->g();
   const size_t per = (size_t)g.ns * g.nr * g.nt;
   HIP_TRY(hipStreamSynchronize(e->stream));
-  if (esum) HIP_TRY(hipMemcpy(esum, e->esum, sizeof(double) * per, hipMemcpyDeviceToHost));
-  if (history && e->hist) {
+  if (esum) HIP_TRY(hipMemcpy(esum, e->t.esum, sizeof(double) * per, hipMemcpyDeviceToHost));
+  if (history && e->t.hist) {
     const int rows = std::min(max_rows, e->hist_rows);
-    if (rows > 0) HIP_TRY(hipMemcpy(history, e->hist, per * rows, hipMemcpyDeviceToHost));
+    if (rows > 0) HIP_TRY(hipMemcpy(history, e->t.hist, per * rows, hipMemcpyDeviceToHost));
   }
-  HIP_TRY(launch_fill_f64(e->esum, 0.0, per, e->stream));
+  HIP_TRY(launch_fill_f64(e->t.esum, 0.0, per, e->stream));
   HIP_TRY(hipStreamSynchronize(e->stream));
   return EA_OK;
 }
@@ -639,5 +761,98 @@ extern "C" int ea_read_timing(ea_engine *e, double *ms, int64_t *launches) {
     e->ms[k] = 0;
     e->launches[k] = 0;
   }
+  return EA_OK;
+}
+
+// ------------------------------------------------------------------ batch
+
+extern "C" int ea_batch_create(ea_engine *const *engines, int n, ea_batch **out) {
+  if (!out || !engines || n < 1) return fail(EA_E_ARG, "invalid argument");
+  *out = nullptr;
+  if (n > (1 << (32 - kMapShift)))
+    return fail(EA_E_ARG, "a batch holds at most %d tubes", 1 << (32 - kMapShift));
+  for (int i = 0; i < n; ++i) {
+    if (!engines[i]) return fail(EA_E_ARG, "engine %d is NULL", i);
+    for (int j = 0; j < i; ++j)
+      if (engines[j] == engines[i]) return fail(EA_E_ARG, "engine %d appears twice", i);
+    if (engines[i]->device != engines[0]->device)
+      return fail(EA_E_ARG, "engines of one batch must share a device");
+  }
+  auto b = std::make_unique<ea_batch>();
+  b->eng.assign(engines, engines + n);
+  b->device = engines[0]->device;
+  HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  RC_TRY(dalloc(&b->dt, (size_t)n));
+  *out = b.release();
+  return EA_OK;
+}
+
+extern "C" int ea_batch_destroy(ea_batch *b) {
+  if (!b) return EA_OK;
+  (void)hipStreamSynchronize(b->stream);
+  delete b;
+  return EA_OK;
+}
+
+static int drop_graph(ea_batch *b) {
+  if (b->exec) HIP_TRY(hipGraphExecDestroy(b->exec));
+  if (b->graph) HIP_TRY(hipGraphDestroy(b->graph));
+  b->exec = nullptr;
+  b->graph = nullptr;
+  return EA_OK;
+}
+
+extern "C" int ea_batch_run(ea_batch *b, const ea_schedule *sch, uint64_t *rng_offsets_out) {
+  if (!b || !sch) return fail(EA_E_ARG, "NULL argument");
+  const int n = (int)b->eng.size();
+  StepShape sh;
+  sh.mesfr = sch[0].mesfr;
+  sh.houfr = sch[0].houfr;
+  sh.ptfr = sch[0].ptfr;
+  int steps = 0;
+  for (int i = 0; i < n; ++i) {
+    const ea_schedule &s = sch[i];
+    if (s.mesfr != sh.mesfr || s.houfr != sh.houfr || s.ptfr != sh.ptfr)
+      return fail(EA_E_ARG, "tube %d: the tubes of a batch share mesfr, houfr and ptfr", i);
+    RC_TRY(check_schedule(b->eng[i], &s));
+    steps = std::max(steps, s.n_it);
+  }
+  std::vector<Tube> tubes(n);
+  std::vector<int> cluster(n);
+  for (int i = 0; i < n; ++i) {
+    ea_engine *e = b->eng[i];
+    RC_TRY(load_schedule(e, &sch[i]));
+    tubes[i] = e->t;
+    cluster[i] = e->t.cluster;
+    sh.cluster = sh.cluster || e->t.cluster;
+    sh.unfused = sh.unfused || !measured_in_pt(e->t);
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipMemcpy(b->dt, tubes.data(), sizeof(Tube) * n, hipMemcpyHostToDevice));
+  if (cluster != b->plan_cluster || !b->plan.buf) {
+    RC_TRY(drop_graph(b));
+    HIP_TRY(upload_plan(make_plan(tubes), &b->plan));
+    b->plan_cluster = cluster;
+  }
+  if (!b->exec || !(sh == b->shape)) {
+    RC_TRY(drop_graph(b));
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const Target x{b->dt, &b->plan, n, b->stream, nullptr};
+    const int rc = enqueue_step(x, sh);
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIP_TRY(ce);
+    b->graph = graph;
+    HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->shape = sh;
+  }
+  for (int k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < n; ++i)
+    finish_schedule(b->eng[i], &sch[i], rng_offsets_out ? rng_offsets_out + i : nullptr);
   return EA_OK;
 }

@@ -89,6 +89,27 @@ __device__ __forceinline__ uint32_t rnd_word(uint32_t x, uint32_t y, uint32_t z,
   return (bb << rot) | (bb >> ((32 - rot) & 31));
 }
 
+// Quotient i / d from m = ceil(2^32 / d) (m = 0 for d = 1): exact for
+// i < 2^26 and d <= 64, one multiply instead of a division sequence.
+__device__ __forceinline__ uint32_t qdiv(uint32_t i, uint32_t m) {
+  return m ? __umulhi(i, m) : i;
+}
+
+// The tube of this block and the block's index within the tube (block map
+// entry tube << kMapShift | local); the tube's step counter is read once.
+struct Where {
+  const Tube *t;
+  int local, step;
+};
+__device__ __forceinline__ Where locate(const Tube *tubes, const uint32_t *map) {
+  const uint32_t e = map[blockIdx.x];
+  Where w;
+  w.t = tubes + (e >> kMapShift);
+  w.local = (int)(e & ((1u << kMapShift) - 1));
+  w.step = *w.t->step;
+  return w;
+}
+
 // Bit permutations bringing a neighbour word whose MSC site wrapped around
 // the sub-lattice to the copy layout of the centre word.
 constexpr uint64_t kMaskE = 0x1111111111111111ULL;
@@ -126,7 +147,7 @@ struct Nbrs {
 };
 
 __device__ __forceinline__ Nbrs neighbours(const Lattice &g, int p, int h) {
-  const int gz = h / g.hs, rem = h - gz * g.hs, by = rem / g.hl;
+  const int gz = qdiv(h, g.mhs), rem = h - gz * g.hs, by = qdiv(rem, g.mhl);
   const int ax = 2 * (rem - by * g.hl) + ((p + by + gz) & 1);
   Nbrs nb;
   nb.gz = gz;
@@ -193,28 +214,34 @@ constexpr int kSwPos = 64;                  // positions per block (one wave)
 constexpr int kSwSlots = 16;                // samples in flight per position
 constexpr int kSwThreads = kSwPos * kSwSlots;
 
-__global__ void __launch_bounds__(kSwThreads) k_sweep(Lattice g, SweepArgs a) {
+__global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *tubes, const uint32_t *map,
+                                                      Phase ph) {
   __shared__ uint32_t lut[EA_LUT_BYTES / 4];
   __shared__ uint8_t clone_at[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
   __shared__ uint32_t words[kSwPos][6];
   __shared__ uint64_t ids[kSwPos][2];
   __shared__ uint32_t ctr[EA_MAX_REPLICAS][5];
-  const int beta = blockIdx.y, tx = threadIdx.x;
+  const Where wh = locate(tubes, map);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int beta = wh.local / a.nbx, bxi = wh.local - beta * a.nbx, tx = threadIdx.x;
   const uint32_t *src =
       reinterpret_cast<const uint32_t *>(a.lut + (size_t)beta * EA_LUT_BYTES);
   for (int i = tx; i < EA_LUT_BYTES / 4; i += kSwThreads) lut[i] = src[i];
   for (int i = tx; i < g.ns * g.nr; i += kSwThreads)
     clone_at[i] = a.wctb[(size_t)i * g.nt + beta];
   if (tx < g.nr) {
-    ctr[tx][0] = a.ctr[tx].x;
-    ctr[tx][1] = a.ctr[tx].y;
-    ctr[tx][2] = a.ctr[tx].z;
-    ctr[tx][3] = a.ctr[tx].w;
+    const ea_counter c = a.ctr[((size_t)wh.step * a.halves + ph.half) * g.nr + tx];
+    ctr[tx][0] = c.x;
+    ctr[tx][1] = c.y;
+    ctr[tx][2] = c.z;
+    ctr[tx][3] = c.w;
     ctr[tx][4] = a.keys[tx];
   }
   __syncthreads();
 
-  const int nwords = g.n * g.nr, p0 = blockIdx.x * kSwPos;
+  const int nwords = g.n * g.nr, p0 = bxi * kSwPos;
   if (tx < 6 * kSwPos) {
     const int pos = tx / 6, k = tx - 6 * pos, t = p0 + pos;
     if (t < nwords) {
@@ -241,7 +268,7 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(Lattice g, SweepArgs a) {
   const bool valid = t < nwords;
   const int h = valid ? t >> g.nr_shift : 0, r = t & (g.nr - 1);
   const uint64_t id1 = ids[pos][0], id2 = ids[pos][1];
-  const int p = a.parity;
+  const int p = ph.half & 1;
   const Nbrs nb = neighbours(g, p, h);
 
   for (int s = slot; s < g.ns; s += kSwSlots) {
@@ -268,19 +295,19 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(Lattice g, SweepArgs a) {
       const uint64_t flip = (j1 & j3 & id1) | ((j24 | id2) & (j1 | j3 | id1)) |
                             (j2 & j4) | (j24 & id2);
       cur[(size_t)h * g.nr + r] = sv ^ flip;
-      if (a.energy) {
+      if (ph.energy) {
         int u = 0;
 #pragma unroll
         for (int d = 0; d < 6; ++d) u += __popcll(f[d] ^ flip);
         e = 6 * 64 - 2 * u;
       }
     }
-    if (a.energy) {
+    if (ph.energy) {
       // every bond has exactly one odd end: the odd half-sweep sees them
       // all; one partial per (sample, replica, beta, block), no atomics
       e = replica_sum(e, g.nr);
       if (pos < g.nr)
-        a.part[(((size_t)s * g.nr + r) * g.nt + beta) * gridDim.x + blockIdx.x] = e;
+        a.part[(((size_t)s * g.nr + r) * g.nt + beta) * a.nbx + bxi] = e;
     }
   }
 }
@@ -289,31 +316,41 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(Lattice g, SweepArgs a) {
 
 // Energy of every (sample, replica, clone) from the even words
 // (d_computeEne, main.cu:1209-1381): sum of 384 - 2 popcount(unsatisfied).
-// Each wave writes one partial per replica; k_energy_sum adds them up.
-__global__ void __launch_bounds__(256) k_energy(Lattice g, const uint64_t *spin,
-                                                const uint64_t *J, int32_t *part) {
-  const int clone = blockIdx.y, s = blockIdx.z;
+// Block (bx, clone, s) of a tube; each wave writes one partial per replica,
+// k_energy_sum adds them up.
+__device__ __forceinline__ int energy_blocks_x(const Lattice &g) {
+  return (g.n * g.nr + 255) / 256;
+}
+
+__global__ void __launch_bounds__(256) k_energy(const Tube *tubes, const uint32_t *map) {
+  const Where wh = locate(tubes, map);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int nbx = energy_blocks_x(g);
+  const int bxi = wh.local % nbx, cs = wh.local / nbx;
+  const int clone = cs % g.nt, s = cs / g.nt;
   const int nwords = g.n * g.nr;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = bxi * blockDim.x + threadIdx.x;
   const bool valid = t < nwords;
   const int h = valid ? t >> g.nr_shift : 0, r = t & (g.nr - 1);
   int e = 0;
   if (valid) {
-    const uint64_t *base = spin + (size_t)s * 2 * g.half + (size_t)clone * nwords;
+    const uint64_t *base = a.spin + (size_t)s * 2 * g.half + (size_t)clone * nwords;
     const uint64_t sv = base[(size_t)h * g.nr + r];
     const Nbrs nb = neighbours(g, 0, h);
     uint64_t sc[6];
     load_neighbours(g, base + g.half, r, nb, 0, sc, g.nr);
-    const uint64_t *Jw = J + ((size_t)(2 * s) * g.n + h) * 6;
+    const uint64_t *Jw = a.J + ((size_t)(2 * s) * g.n + h) * 6;
     int u = 0;
 #pragma unroll
     for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
     e = 6 * 64 - 2 * u;
   }
   e = replica_sum(e, g.nr);
-  const int lane = threadIdx.x & 63, nw = gridDim.x * (blockDim.x / 64);
-  const int wv = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (lane < g.nr) part[(((size_t)s * g.nr + r) * g.nt + clone) * nw + wv] = e;
+  const int lane = threadIdx.x & 63, nw = 4 * nbx;
+  const int wv = bxi * 4 + (threadIdx.x >> 6);
+  if (lane < g.nr) a.part[(((size_t)s * g.nr + r) * g.nt + clone) * nw + wv] = e;
 }
 
 __device__ __forceinline__ int wave_sum(int v) {
@@ -341,18 +378,21 @@ __device__ __forceinline__ void sum_partials(const int32_t *part, int nt, int nw
   }
 }
 
-// ener[chain][c] from the chain's partials: slot c itself (by_beta NULL),
-// or slot b whose clone is c = by_beta[chain][b] (the sweep's partials).
-__global__ void __launch_bounds__(64) k_energy_sum(Lattice g, const int32_t *part,
-                                                   int nw, const uint8_t *by_beta,
-                                                   int32_t *ener) {
+// ener[chain][c] from the chain's partials: the sweep's partials by beta
+// (BY_BETA, clone c = wctb[chain][b]) or k_energy's partials by clone.
+template <bool BY_BETA>
+__global__ void __launch_bounds__(64) k_energy_sum(const Tube *tubes, const uint32_t *map) {
   __shared__ int32_t E[EA_MAX_BETAS];
-  const int chain = blockIdx.x, nt = g.nt, lane = threadIdx.x;
-  sum_partials(part + (size_t)chain * nt * nw, nt, nw, lane, E);
+  const Where wh = locate(tubes, map);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const int chain = wh.local, nt = a.g.nt, lane = threadIdx.x;
+  const int nw = BY_BETA ? a.nbx : 4 * energy_blocks_x(a.g);
+  sum_partials(a.part + (size_t)chain * nt * nw, nt, nw, lane, E);
   __syncthreads();
   for (int b = lane; b < nt; b += 64) {
-    const int c = by_beta ? by_beta[(size_t)chain * nt + b] : b;
-    ener[(size_t)chain * nt + c] = E[b];
+    const int c = BY_BETA ? a.wctb[(size_t)chain * nt + b] : b;
+    a.ener[(size_t)chain * nt + c] = E[b];
   }
 }
 
@@ -360,11 +400,11 @@ __global__ void __launch_bounds__(64) k_energy_sum(Lattice g, const int32_t *par
 
 // exp(pt) of the host's libm for pair i and energy difference d < 0: the
 // tabulated value, 0 where exp underflows.
-__device__ __forceinline__ double exact_exp(const PtArgs &a, int i, int d) {
+__device__ __forceinline__ double exact_exp(const Tube &a, int i, int d) {
   // energies are sums of 384 - 2*popcount terms, so d is even and k >= 1
   const int k = -d / 2;
   if (k < 1) return exp(a.nd[i] * (double)d);
-  return k <= a.len[i] ? a.tab[a.off[i] + k - 1] : 0.0;
+  return k <= a.tlen[i] ? a.tab[a.toff[i] + k - 1] : 0.0;
 }
 
 // Parallel_Tempering (upd.c:3-48), one wave per (sample, replica) chain.
@@ -378,7 +418,7 @@ __device__ __forceinline__ double exact_exp(const PtArgs &a, int i, int d) {
 // Both sides' errors are far below 1e-12, so every decision equals the
 // reference's.  The statistics are then added by one lane per pair with the
 // tabulated exp values.
-__global__ void __launch_bounds__(64) k_pt(Lattice g, PtArgs a) {
+__global__ void __launch_bounds__(64) k_pt(const Tube *tubes, const uint32_t *map, Phase ph) {
   __shared__ uint64_t ira[256];
   __shared__ uint64_t zs[EA_MAX_BETAS], prs[EA_MAX_BETAS];
   __shared__ double uu[EA_MAX_BETAS], lu[EA_MAX_BETAS];
@@ -386,11 +426,15 @@ __global__ void __launch_bounds__(64) k_pt(Lattice g, PtArgs a) {
   __shared__ uint8_t perm[EA_MAX_BETAS + 1];
   __shared__ int32_t dE[EA_MAX_BETAS];
   __shared__ uint8_t took[EA_MAX_BETAS];
-  const int chain = blockIdx.x, nt = g.nt, np = nt - 1, lane = threadIdx.x;
-  ea_hq_state *st = a.st + chain;
+  const Where wh = locate(tubes, map);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int chain = wh.local, nt = g.nt, np = nt - 1, lane = threadIdx.x;
+  ea_hq_state *st = a.pt + chain;
   for (int i = lane; i < 256; i += 64) ira[i] = st->ira[i];
   for (int i = lane; i < nt; i += 64) perm[i] = a.wctb[(size_t)chain * nt + i];
-  if (a.part) {
+  if (ph.energy) {
     // the sweep left partial energies by beta: dE holds them by beta for a
     // moment, E receives them by clone
     sum_partials(a.part + (size_t)chain * nt * a.nbx, nt, a.nbx, lane, dE);
@@ -467,12 +511,15 @@ __global__ void __launch_bounds__(64) k_pt(Lattice g, PtArgs a) {
   __syncthreads();
   for (int i = lane; i < 256; i += 64) st->ira[i] = ira[i];
   for (int i = lane; i < nt; i += 64) a.wctb[(size_t)chain * nt + i] = perm[i];
-  if (a.measure)  // per-beta energy sums and history (main.cu:208-228)
+  if (ph.measure && measured_in_pt(a)) {  // energy sums, history (main.cu:208-228)
+    const int itb = a.it0 + wh.step;
+    const int row = itb % a.frec == a.frec - 1 ? itb / a.frec : -1;
     for (int b = lane; b < nt; b += 64) {
       const int c = perm[b];
       a.esum[(size_t)chain * nt + b] += (double)E[c];
-      if (a.row >= 0) a.hist[((size_t)a.row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
+      if (row >= 0) a.hist[((size_t)row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
     }
+  }
   for (int i = lane; i < np; i += 64) {
     const size_t k = (size_t)chain * np + i;
     const double pt = a.nd[i] * (double)dE[i];
@@ -485,14 +532,25 @@ __global__ void __launch_bounds__(64) k_pt(Lattice g, PtArgs a) {
 // --------------------------------------------------------------- measures
 
 // Per-beta energy sums and the beta-of-clone history (main.cu:208-228).
-__global__ void k_measure(Lattice g, MeasureArgs a) {
-  const int chain = blockIdx.x, nt = g.nt;
+__global__ void __launch_bounds__(64) k_measure(const Tube *tubes, const uint32_t *map) {
+  const Where wh = locate(tubes, map);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int chain = wh.local, nt = g.nt;
+  const int itb = a.it0 + wh.step;
+  const int row = itb % a.frec == a.frec - 1 ? itb / a.frec : -1;
   for (int b = threadIdx.x; b < nt; b += blockDim.x) {
     const int c = a.wctb[(size_t)chain * nt + b];
     a.esum[(size_t)chain * nt + b] += (double)a.ener[(size_t)chain * nt + c];
-    if (a.row >= 0)
-      a.hist[((size_t)a.row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
+    if (row >= 0) a.hist[((size_t)row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
   }
+}
+
+// Advances the step counter of every tube that has steps left.
+__global__ void k_step_end(const Tube *tubes, int ntubes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ntubes && *tubes[i].step < tubes[i].nsteps) *tubes[i].step += 1;
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
@@ -528,13 +586,32 @@ __device__ __forceinline__ uint32_t coin_word(uint64_t seed, uint64_t idx0) {
   }
 }
 
+// A Houdayer instance (group, sample, beta < ctm) of a tube: local index
+// beta + ctm * (2 * s + grp), and the inputs of the call in flight.
+struct Inst {
+  int beta, s, grp;
+  const int32_t *pairs;  // r1, r2 of group 0, then of group 1
+  uint64_t coin;         // coin index of (this group, sample 0, beta 0, site 0)
+};
+__device__ __forceinline__ Inst instance_of(const Tube &a, int local, int step, int call) {
+  Inst in;
+  in.beta = local % a.g.ctm;
+  const int sg = local / a.g.ctm;
+  in.s = sg >> 1;
+  in.grp = sg & 1;
+  const uint64_t k = (uint64_t)step * a.calls + call;
+  in.pairs = a.pairs + k * 4;
+  in.coin = a.coin0 + k * a.coin_per_call + (uint64_t)in.grp * a.g.V * a.g.nt * a.g.ns;
+  return in;
+}
+
 // Word offsets of replicas r1, r2 of the instance (group, sample, beta).
 struct Pair {
   size_t o1, o2;
 };
-__device__ __forceinline__ Pair pair_words(const Lattice &g, const HouArgs &a,
+__device__ __forceinline__ Pair pair_words(const Lattice &g, const Tube &a, const int32_t *pairs,
                                            int s, int grp, int beta) {
-  const int r1 = a.pairs[2 * grp], r2 = a.pairs[2 * grp + 1];
+  const int r1 = pairs[2 * grp], r2 = pairs[2 * grp + 1];
   const int c1 = a.wctb[((size_t)s * g.nr + r1) * g.nt + beta];
   const int c2 = a.wctb[((size_t)s * g.nr + r2) * g.nt + beta];
   const size_t sb = (size_t)s * 2 * g.half;
@@ -547,7 +624,7 @@ __device__ __forceinline__ Pair pair_words(const Lattice &g, const HouArgs &a,
 // Linear site index (x + L y + L^2 z) of copy 0 of the parity-p site at half
 // index h; copy `bit` adds copy_offset(bit).
 __device__ __forceinline__ int site_base(const Lattice &g, int p, int h) {
-  const int gz = h / g.hs, rem = h - gz * g.hs, by = rem / g.hl;
+  const int gz = qdiv(h, g.mhs), rem = h - gz * g.hs, by = qdiv(rem, g.mhl);
   const int ax = 2 * (rem - by * g.hl) + ((p + by + gz) & 1);
   return ax + g.L * (by + g.L * gz);
 }
@@ -598,12 +675,6 @@ __device__ __forceinline__ void unite(uint32_t *lab, uint32_t a, uint32_t b) {
     if (old == a) return;
     a = old;
   }
-}
-
-// Quotient i / d from m = ceil(2^32 / d) (m = 0 for d = 1): exact for
-// i < 2^26 and d <= 64, one multiply instead of a division sequence.
-__device__ __forceinline__ uint32_t qdiv(uint32_t i, uint32_t m) {
-  return m ? __umulhi(i, m) : i;
 }
 
 // Union-find forest in LDS, parents as 32-bit node indices.  Loads and stores
@@ -784,9 +855,15 @@ __device__ __forceinline__ uint64_t word_move_bits(const Lattice &g, const uint6
 // Dynamic LDS: rows[V/L] u64 | qw[2n] u64 | nw[2n] u64 | cb[V/32] u32 (coin
 // of every site) | forest[V] u32.
 template <typename RowT>
-__global__ void __launch_bounds__(1024) k_hou_whole(Lattice g, HouArgs a) {
+__global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uint32_t *map,
+                                                    Phase ph) {
   using R = Rows<RowT>;
   extern __shared__ uint64_t lds[];
+  const Where wh = locate(tubes, map);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const Inst in = instance_of(a, wh.local, wh.step, ph.call);
   const int V = g.V, L = g.L, nrows = V / L, T = blockDim.x, tid = threadIdx.x;
   uint64_t *rows = lds;         // q rows, then move rows
   uint64_t *qw = rows + nrows;  // q words, then replica r1's new words
@@ -794,8 +871,8 @@ __global__ void __launch_bounds__(1024) k_hou_whole(Lattice g, HouArgs a) {
   uint32_t *cb = reinterpret_cast<uint32_t *>(nw + 2 * g.n);
   const Forest f{cb + V / 32};
   __shared__ int esum[2];
-  const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
-  const Pair pw = pair_words(g, a, s, grp, beta);
+  const int beta = in.beta, s = in.s, grp = in.grp;
+  const Pair pw = pair_words(g, a, in.pairs, s, grp, beta);
   if (tid < 2) esum[tid] = 0;
   for (int w = tid; w < 2 * g.n; w += T) {
     const int p = w >= g.n, h = w - p * g.n;
@@ -830,20 +907,19 @@ __global__ void __launch_bounds__(1024) k_hou_whole(Lattice g, HouArgs a) {
       link(r - L * (g.Lz - 1));
   }
   // coins of 128 consecutive sites per Philox call
-  const uint64_t cidx = a.offset0 + (uint64_t)grp * V * g.nt * g.ns +
-                        (uint64_t)s * V * g.nt + (uint64_t)beta * V;
+  const uint64_t cidx = in.coin + (uint64_t)s * V * g.nt + (uint64_t)beta * V;
   for (int c = tid; c < V / 128; c += T) {
     const uint64_t blk = (cidx >> 7) + c;
     uint32_t v0 = (uint32_t)blk, v1 = (uint32_t)(blk >> 32), v2 = 0x486f7564u, v3 = 0;
-    philox10(v0, v1, v2, v3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+    philox10(v0, v1, v2, v3, (uint32_t)a.coin_seed, (uint32_t)(a.coin_seed >> 32));
     cb[4 * c] = v0;
     cb[4 * c + 1] = v1;
     cb[4 * c + 2] = v2;
     cb[4 * c + 3] = v3;
   }
   __syncthreads();
-  if (a.labels_out) {
-    uint32_t *out = a.labels_out + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
+  if (a.keep) {
+    uint32_t *out = a.keep + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
     for (int i = tid; i < V; i += T) {
       const int r = qdiv(i, g.mL), x = i - r * L;
       out[i] = f.root(r * L + R::rep(R::bounds((RowT)rows[r], L, full), x));
@@ -923,100 +999,131 @@ __global__ void __launch_bounds__(1024) k_hou_whole(Lattice g, HouArgs a) {
   }
   __syncthreads();
   if (tid < 2) {
-    const int r = a.pairs[2 * grp + tid];
+    const int r = in.pairs[2 * grp + tid];
     const size_t chain = (size_t)s * g.nr + r;
     const int c = a.wctb[chain * g.nt + beta];
     const int old = a.ener[chain * g.nt + c];
     a.ener[chain * g.nt + c] = esum[tid];
     // the sums already hold the pre-move energy; all values are integers
     // below 2^53, so adding the difference is exact
-    if (a.measure) a.esum[chain * g.nt + beta] += (double)(esum[tid] - old);
+    if (ph.measure) a.esum[chain * g.nt + beta] += (double)(esum[tid] - old);
   }
 }
 
 // Tiled Houdayer for tubes larger than one workgroup's LDS.  Pass 1 labels
 // each slab of `planes` z-planes in LDS and stores global labels; pass 2
-// unites across slab boundaries in global memory; pass 3 flattens and draws
-// the root coins; pass 4 applies the move.
-__device__ __forceinline__ uint32_t q_site(const Lattice &g, const HouArgs &a,
+// unites across slab boundaries in global memory; pass 3 flattens; pass 4
+// applies the move.  Slab block local index: slab + nslabs * instance.
+__device__ __forceinline__ uint32_t q_site(const Lattice &g, const Tube &a,
                                            const Pair &pw, int site) {
-  const int x = site % g.L, yz = site / g.L, y = yz % g.L, z = yz / g.L;
+  const int yz = qdiv(site, g.mL), x = site - yz * g.L;
+  const int z = qdiv(yz, g.mL), y = yz - z * g.L;
   const int p = (x ^ y ^ z) & 1;
-  const int bit = x / g.ML + 4 * (y / g.ML) + 16 * (z / g.MLz);
-  const int h = ((x % g.ML) + g.ML * (y % g.ML) + g.MS * (z % g.MLz)) >> 1;
+  const int bx = qdiv(x, g.mML), by = qdiv(y, g.mML), bz = qdiv(z, g.mMLz);
+  const int bit = bx + 4 * by + 16 * bz;
+  const int h = ((x - bx * g.ML) + g.ML * (y - by * g.ML) + g.MS * (z - bz * g.MLz)) >> 1;
   const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
   return (uint32_t)(((a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o]) >> bit) & 1);
 }
 
-__device__ __forceinline__ size_t instance(const Lattice &g, int grp, int s,
-                                           int beta) {
-  return ((size_t)grp * g.ns + s) * g.ctm + beta;
+__device__ __forceinline__ size_t instance_index(const Lattice &g, const Inst &in) {
+  return ((size_t)in.grp * g.ns + in.s) * g.ctm + in.beta;
+}
+
+__device__ __forceinline__ int slab_count(const Tube &a) {
+  return (a.g.Lz + a.planes - 1) / a.planes;
 }
 
 // Dynamic LDS: rows[planes*L] u64 | lab[planes*L*L] u32.
-__global__ void __launch_bounds__(1024) k_hou_slab(Lattice g, HouArgs a) {
+__global__ void __launch_bounds__(1024) k_hou_slab(const Tube *tubes, const uint32_t *map,
+                                                   Phase ph) {
   extern __shared__ uint64_t lds[];
-  const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
-  const int S = g.L * g.L, z0 = blockIdx.x * a.planes;
+  const Where wh = locate(tubes, map);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int ns = slab_count(a), slab = wh.local % ns;
+  const Inst in = instance_of(a, wh.local / ns, wh.step, ph.call);
+  const int S = g.L * g.L, z0 = slab * a.planes;
   const int planes = min(a.planes, g.Lz - z0), nsite = planes * S;
   const int T = blockDim.x, tid = threadIdx.x;
   uint64_t *rows = lds;
   uint32_t *lab = reinterpret_cast<uint32_t *>(rows + a.planes * g.L);
-  const Pair pw = pair_words(g, a, s, grp, beta);
-  const size_t ib = instance(g, grp, s, beta);
+  const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
+  const size_t ib = instance_index(g, in);
   const int gbase = z0 * S;
   for (int r = tid; r < planes * g.L; r += T)
     rows[r] = row_bits(g, z0 * g.L + r, [&](int p, int h) {
       const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
       return a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
     });
-  const uint64_t cidx = a.offset0 + (uint64_t)grp * g.V * g.nt * g.ns +
-                        (uint64_t)s * g.V * g.nt + (uint64_t)beta * g.V + gbase;
+  const uint64_t cidx = in.coin + (uint64_t)in.s * g.V * g.nt + (uint64_t)in.beta * g.V + gbase;
   for (int c = tid; c < nsite / 32; c += T)
-    a.cbits[ib * (g.V / 32) + gbase / 32 + c] = coin_word(a.seed, cidx + 32 * c);
+    a.hcb[ib * (g.V / 32) + gbase / 32 + c] = coin_word(a.coin_seed, cidx + 32 * c);
   __syncthreads();
   label_rows(g, lab, rows, planes, false);
-  uint32_t *gl = a.lab + ib * g.V + gbase;
+  uint32_t *gl = a.hlab + ib * g.V + gbase;
   for (int i = tid; i < nsite; i += T) gl[i] = gbase + lab[i];
 }
 
 // z bonds between plane z0-1 (or Lz-1 for the periodic wrap) and plane z0,
-// the first plane of slab blockIdx.x.
-__global__ void __launch_bounds__(256) k_hou_seam(Lattice g, HouArgs a) {
-  const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
-  const int S = g.L * g.L, z0 = blockIdx.x * a.planes;
+// the first plane of the slab.
+__global__ void __launch_bounds__(256) k_hou_seam(const Tube *tubes, const uint32_t *map,
+                                                  Phase ph) {
+  const Where wh = locate(tubes, map);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int ns = slab_count(a), slab = wh.local % ns;
+  const Inst in = instance_of(a, wh.local / ns, wh.step, ph.call);
+  const int S = g.L * g.L, z0 = slab * a.planes;
   int zl = z0 - 1;
   if (z0 == 0) {
     if (g.bc != EA_PBC) return;
     zl = g.Lz - 1;
   }
-  const Pair pw = pair_words(g, a, s, grp, beta);
-  uint32_t *gl = a.lab + instance(g, grp, s, beta) * g.V;
+  const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
+  uint32_t *gl = a.hlab + instance_index(g, in) * g.V;
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     const int u = z0 * S + i, d = zl * S + i;
     if (q_site(g, a, pw, u) == q_site(g, a, pw, d)) unite(gl, u, d);
   }
 }
 
-__global__ void __launch_bounds__(256) k_hou_flatten(Lattice g, HouArgs a) {
-  const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
-  const size_t ib = instance(g, grp, s, beta);
-  uint32_t *gl = a.lab + ib * g.V;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ int site_blocks(const Lattice &g) { return (g.V + 255) / 256; }
+__device__ __forceinline__ int word_blocks(const Lattice &g) { return (2 * g.n + 255) / 256; }
+
+__global__ void __launch_bounds__(256) k_hou_flatten(const Tube *tubes, const uint32_t *map,
+                                                     Phase ph) {
+  const Where wh = locate(tubes, map);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int nb = site_blocks(g), bxi = wh.local % nb;
+  const Inst in = instance_of(a, wh.local / nb, wh.step, ph.call);
+  const size_t ib = instance_index(g, in);
+  uint32_t *gl = a.hlab + ib * g.V;
+  const int i = bxi * blockDim.x + threadIdx.x;
   if (i >= g.V) return;
   const uint32_t root = find_root(gl, i);
   gl[i] = root;
-  if (a.labels_out) a.labels_out[ib * g.V + i] = root;
+  if (a.keep) a.keep[ib * g.V + i] = root;
 }
 
-__global__ void __launch_bounds__(256) k_hou_apply(Lattice g, HouArgs a) {
-  const int beta = blockIdx.y, s = blockIdx.z >> 1, grp = blockIdx.z & 1;
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_hou_apply(const Tube *tubes, const uint32_t *map,
+                                                   Phase ph) {
+  const Where wh = locate(tubes, map);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int nb = word_blocks(g), bxi = wh.local % nb;
+  const Inst in = instance_of(a, wh.local / nb, wh.step, ph.call);
+  const int w = bxi * blockDim.x + threadIdx.x;
   if (w >= 2 * g.n) return;
-  const size_t ib = instance(g, grp, s, beta);
-  const uint32_t *gl = a.lab + ib * g.V;
-  const uint32_t *cb = a.cbits + ib * (g.V / 32);
-  const Pair pw = pair_words(g, a, s, grp, beta);
+  const size_t ib = instance_index(g, in);
+  const uint32_t *gl = a.hlab + ib * g.V;
+  const uint32_t *cb = a.hcb + ib * (g.V / 32);
+  const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
   const int p = w >= g.n, h = w - p * g.n;
   const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
   const int base = site_base(g, p, h);
@@ -1039,38 +1146,11 @@ inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) /
 
 int sweep_blocks_x(const Lattice &g) { return (int)blocks_for((long long)g.n * g.nr, kSwPos); }
 
+static int energy_blocks_host(const Lattice &g) { return (int)blocks_for((long long)g.n * g.nr, 256); }
+
 size_t energy_partials(const Lattice &g) {
   const size_t per = (size_t)g.ns * g.nr * g.nt;
-  const size_t sweep = per * sweep_blocks_x(g);
-  const size_t energy = per * 4 * blocks_for((long long)g.n * g.nr, 256);
-  return std::max(sweep, energy);
-}
-
-hipError_t launch_sweep(const Lattice &g, const SweepArgs &a, hipStream_t s) {
-  dim3 grid(sweep_blocks_x(g), g.nt);
-  hipLaunchKernelGGL(k_sweep, grid, dim3(kSwThreads), 0, s, g, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_energy(const Lattice &g, const uint64_t *spin, const uint64_t *J,
-                         int32_t *ener, int32_t *part, hipStream_t s) {
-  const unsigned bx = blocks_for((long long)g.n * g.nr, 256);
-  hipLaunchKernelGGL(k_energy, dim3(bx, g.nt, g.ns), dim3(256), 0, s, g, spin, J, part);
-  hipLaunchKernelGGL(k_energy_sum, dim3(g.ns * g.nr), dim3(64), 0, s, g,
-                     (const int32_t *)part, (int)(4 * bx), (const uint8_t *)nullptr, ener);
-  return hipGetLastError();
-}
-
-hipError_t launch_resolve(const Lattice &g, const int32_t *part, const uint8_t *wctb,
-                          int32_t *ener, hipStream_t s) {
-  hipLaunchKernelGGL(k_energy_sum, dim3(g.ns * g.nr), dim3(64), 0, s, g, part,
-                     sweep_blocks_x(g), wctb, ener);
-  return hipGetLastError();
-}
-
-hipError_t launch_pt(const Lattice &g, const PtArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_pt, dim3(g.ns * g.nr), dim3(64), 0, s, g, a);
-  return hipGetLastError();
+  return per * std::max<size_t>(sweep_blocks_x(g), 4 * (size_t)energy_blocks_host(g));
 }
 
 size_t hou_whole_lds(const Lattice &g) {
@@ -1080,6 +1160,116 @@ size_t hou_whole_lds(const Lattice &g) {
 
 size_t hou_slab_lds(const Lattice &g, int planes) {
   return 8 * (size_t)planes * g.L + 4 * (size_t)planes * g.L * g.L;
+}
+
+Plan make_plan(const std::vector<Tube> &tubes) {
+  Plan p;
+  for (size_t ti = 0; ti < tubes.size(); ++ti) {
+    const Tube &t = tubes[ti];
+    const Lattice &g = t.g;
+    const uint32_t tag = (uint32_t)ti << kMapShift;
+    auto add = [&](int kind, long long count) {
+      for (long long i = 0; i < count; ++i) p.map[kind].push_back(tag | (uint32_t)i);
+    };
+    add(kMapSweep, (long long)sweep_blocks_x(g) * g.nt);
+    add(kMapEnergy, (long long)energy_blocks_host(g) * g.nt * g.ns);
+    add(kMapChain, (long long)g.ns * g.nr);
+    if (!t.cluster || g.ctm <= 0) continue;
+    if (!measured_in_pt(t)) {
+      add(kMapEnergyU, (long long)energy_blocks_host(g) * g.nt * g.ns);
+      add(kMapChainU, (long long)g.ns * g.nr);
+    }
+    const long long inst = 2LL * g.ns * g.ctm;
+    if (t.planes == 0) {
+      if (g.L <= 32) {
+        add(kMapWhole32, inst);
+        p.lds32 = std::max(p.lds32, hou_whole_lds(g));
+      } else {
+        add(kMapWhole64, inst);
+        p.lds64 = std::max(p.lds64, hou_whole_lds(g));
+      }
+    } else {
+      const long long slabs = (g.Lz + t.planes - 1) / t.planes;
+      add(kMapSlab, slabs * inst);
+      add(kMapSeam, slabs * inst);
+      add(kMapFlatten, (long long)blocks_for(g.V, 256) * inst);
+      add(kMapApply, (long long)blocks_for(2LL * g.n, 256) * inst);
+      p.lds_slab = std::max(p.lds_slab, hou_slab_lds(g, t.planes));
+    }
+  }
+  return p;
+}
+
+hipError_t upload_plan(const Plan &p, DevPlan *d) {
+  free_plan(d);
+  size_t total = 0;
+  for (int k = 0; k < kMapKinds; ++k) {
+    d->off[k] = total;
+    d->cnt[k] = p.map[k].size();
+    total += p.map[k].size();
+  }
+  d->lds32 = p.lds32;
+  d->lds64 = p.lds64;
+  d->lds_slab = p.lds_slab;
+  if (!total) return hipSuccess;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&d->buf), total * sizeof(uint32_t));
+  if (e != hipSuccess) return e;
+  for (int k = 0; k < kMapKinds; ++k)
+    if (d->cnt[k]) {
+      e = hipMemcpy(d->buf + d->off[k], p.map[k].data(), d->cnt[k] * sizeof(uint32_t),
+                    hipMemcpyHostToDevice);
+      if (e != hipSuccess) return e;
+    }
+  return hipSuccess;
+}
+
+void free_plan(DevPlan *d) {
+  if (d->buf) (void)hipFree(d->buf);
+  *d = DevPlan{};
+}
+
+hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s) {
+  if (p.cnt[kMapSweep])
+    hipLaunchKernelGGL(k_sweep, dim3((unsigned)p.cnt[kMapSweep]), dim3(kSwThreads), 0, s,
+                       tubes, p.map(kMapSweep), ph);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(const Tube *tubes, const DevPlan &p, bool unfused_only, hipStream_t s) {
+  const int me = unfused_only ? kMapEnergyU : kMapEnergy, mc = unfused_only ? kMapChainU : kMapChain;
+  if (p.cnt[me])
+    hipLaunchKernelGGL(k_energy, dim3((unsigned)p.cnt[me]), dim3(256), 0, s, tubes, p.map(me));
+  if (p.cnt[mc])
+    hipLaunchKernelGGL(k_energy_sum<false>, dim3((unsigned)p.cnt[mc]), dim3(64), 0, s, tubes,
+                       p.map(mc));
+  return hipGetLastError();
+}
+
+hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, hipStream_t s) {
+  if (p.cnt[kMapChain])
+    hipLaunchKernelGGL(k_energy_sum<true>, dim3((unsigned)p.cnt[kMapChain]), dim3(64), 0, s,
+                       tubes, p.map(kMapChain));
+  return hipGetLastError();
+}
+
+hipError_t launch_pt(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s) {
+  if (p.cnt[kMapChain])
+    hipLaunchKernelGGL(k_pt, dim3((unsigned)p.cnt[kMapChain]), dim3(64), 0, s, tubes,
+                       p.map(kMapChain), ph);
+  return hipGetLastError();
+}
+
+hipError_t launch_measure(const Tube *tubes, const DevPlan &p, bool unfused_only, hipStream_t s) {
+  const int mc = unfused_only ? kMapChainU : kMapChain;
+  if (p.cnt[mc])
+    hipLaunchKernelGGL(k_measure, dim3((unsigned)p.cnt[mc]), dim3(64), 0, s, tubes, p.map(mc));
+  return hipGetLastError();
+}
+
+hipError_t launch_step_end(const Tube *tubes, int ntubes, hipStream_t s) {
+  if (ntubes > 0)
+    hipLaunchKernelGGL(k_step_end, dim3(blocks_for(ntubes, 64)), dim3(64), 0, s, tubes, ntubes);
+  return hipGetLastError();
 }
 
 static void allow_lds(const void *kernel, size_t bytes) {
@@ -1100,34 +1290,28 @@ static void allow_lds(const void *kernel, size_t bytes) {
   (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t launch_houdayer(const Lattice &g, const HouArgs &a, hipStream_t s) {
-  if (g.ctm <= 0) return hipSuccess;
-  const dim3 inst(1, g.ctm, 2 * g.ns);
-  if (a.planes <= 0) {
-    const size_t lds = hou_whole_lds(g);
-    if (g.L <= 32) {
-      allow_lds(reinterpret_cast<const void *>(k_hou_whole<uint32_t>), lds);
-      hipLaunchKernelGGL(k_hou_whole<uint32_t>, inst, dim3(1024), lds, s, g, a);
-    } else {
-      allow_lds(reinterpret_cast<const void *>(k_hou_whole<uint64_t>), lds);
-      hipLaunchKernelGGL(k_hou_whole<uint64_t>, inst, dim3(1024), lds, s, g, a);
-    }
-    return hipGetLastError();
+hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s) {
+  if (p.cnt[kMapWhole32]) {
+    allow_lds(reinterpret_cast<const void *>(k_hou_whole<uint32_t>), p.lds32);
+    hipLaunchKernelGGL(k_hou_whole<uint32_t>, dim3((unsigned)p.cnt[kMapWhole32]), dim3(1024),
+                       p.lds32, s, tubes, p.map(kMapWhole32), ph);
   }
-  const int slabs = (g.Lz + a.planes - 1) / a.planes;
-  const size_t lds = hou_slab_lds(g, a.planes);
-  allow_lds(reinterpret_cast<const void *>(k_hou_slab), lds);
-  hipLaunchKernelGGL(k_hou_slab, dim3(slabs, g.ctm, 2 * g.ns), dim3(1024), lds, s, g, a);
-  hipLaunchKernelGGL(k_hou_seam, dim3(slabs, g.ctm, 2 * g.ns), dim3(256), 0, s, g, a);
-  hipLaunchKernelGGL(k_hou_flatten, dim3(blocks_for(g.V, 256), g.ctm, 2 * g.ns),
-                     dim3(256), 0, s, g, a);
-  hipLaunchKernelGGL(k_hou_apply, dim3(blocks_for(2LL * g.n, 256), g.ctm, 2 * g.ns),
-                     dim3(256), 0, s, g, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_measure(const Lattice &g, const MeasureArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_measure, dim3(g.ns * g.nr), dim3(64), 0, s, g, a);
+  if (p.cnt[kMapWhole64]) {
+    allow_lds(reinterpret_cast<const void *>(k_hou_whole<uint64_t>), p.lds64);
+    hipLaunchKernelGGL(k_hou_whole<uint64_t>, dim3((unsigned)p.cnt[kMapWhole64]), dim3(1024),
+                       p.lds64, s, tubes, p.map(kMapWhole64), ph);
+  }
+  if (p.cnt[kMapSlab]) {
+    allow_lds(reinterpret_cast<const void *>(k_hou_slab), p.lds_slab);
+    hipLaunchKernelGGL(k_hou_slab, dim3((unsigned)p.cnt[kMapSlab]), dim3(1024), p.lds_slab, s,
+                       tubes, p.map(kMapSlab), ph);
+    hipLaunchKernelGGL(k_hou_seam, dim3((unsigned)p.cnt[kMapSeam]), dim3(256), 0, s, tubes,
+                       p.map(kMapSeam), ph);
+    hipLaunchKernelGGL(k_hou_flatten, dim3((unsigned)p.cnt[kMapFlatten]), dim3(256), 0, s,
+                       tubes, p.map(kMapFlatten), ph);
+    hipLaunchKernelGGL(k_hou_apply, dim3((unsigned)p.cnt[kMapApply]), dim3(256), 0, s, tubes,
+                       p.map(kMapApply), ph);
+  }
   return hipGetLastError();
 }
 

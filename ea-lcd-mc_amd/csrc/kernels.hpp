@@ -1,16 +1,22 @@
 // kernels.hpp -- launch interface of the gfx950 kernels (internal).
+//
+// Every kernel works on a table of tubes: a block reads a 32-bit entry of a
+// block map (tube number << kMapShift | block index within the tube), then
+// that tube's descriptor.  A single engine is a table of one tube; a batch
+// is a table of many tubes of different sizes advancing in the same launches.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "eatube.h"
 
 namespace eatube {
 
-// Geometry and sizes, passed by value to every kernel.
+// Geometry and sizes of one tube.
 struct Lattice {
   int L, Lz, ML, MLz, MS;  // MSC sub-lattice sides (L/4, Lz/4), MS = ML*ML
   int n, V;                // half MSC volume V/128, sites
@@ -33,73 +39,98 @@ inline void set_divisors(Lattice &g) {
   g.mhs = divisor_magic(g.hs);
 }
 
-// One half-sweep.  Counters and keys travel by value, as the reference's
-// s_time / s_keys kernel arguments do (main.cu:299-305).
-struct SweepArgs {
-  uint64_t *spin;        // [ns][2][nt][n][nr]
-  const uint64_t *J;     // [ns][2][n][6]
-  const uint8_t *wctb;   // [ns][nr][nt] clone at beta
-  const uint8_t *lut;    // [nt][EA_LUT_BYTES]
-  int32_t *part;         // [ns][nr][nt][gridDim.x] partial energies by beta
-  int parity;
-  int energy;            // odd half-sweep: write the energy partials
-  ea_counter ctr[EA_MAX_REPLICAS];
+// Everything the kernels need about one tube: geometry, device state and the
+// inputs of the calls in flight.  Per-call inputs are indexed by the tube's
+// device step counter, so one captured step can be replayed: step k reads
+// counter blocks ctr[k][half][nr], replica pairs pairs[k][call][4] and coin
+// offset coin0 + (k * calls + call) * coin_per_call.
+struct Tube {
+  Lattice g;
+  uint64_t *spin;           // [ns][2][nt][n][nr]
+  const uint64_t *J;        // [ns][2][n][6]
+  uint8_t *wctb;            // [ns][nr][nt] clone at beta
+  const uint8_t *lut;       // [nt][EA_LUT_BYTES]
+  int32_t *ener;            // [ns][nr][nt] by clone
+  int32_t *part;            // partial energies, energy_partials(g) slots
+  ea_hq_state *pt;          // [ns][nr]
+  double *acc, *att, *raw;  // [ns][nr][nt-1]
+  const double *nd;         // [nt-1]  -(beta[i+1]-beta[i])
+  const double *tab;        // exp tables (host libm values)
+  const int32_t *toff, *tlen;
+  double *esum;             // [ns][nr][nt] by beta
+  uint8_t *hist;            // [rows][ns][nr][nt] beta of clone
+  uint32_t *hlab, *hcb;     // slab Houdayer: labels, root coins
+  uint32_t *keep;           // copy of the last labels, or NULL
+  int planes;               // slab Houdayer planes; 0: whole tube in one block
+  int nbx;                  // sweep blocks per beta
+  int cluster;              // Houdayer moves are part of this tube's steps
+  // calls in flight
+  int *step;                // device step counter
+  int nsteps;               // blocks of a tube with *step >= nsteps do nothing
+  int halves, calls;        // half-sweeps and Houdayer calls per step
+  int it0, frec;            // history row of step k: (it0 + k) / frec when
+                            // (it0 + k) % frec == frec - 1
+  const ea_counter *ctr;
+  const int32_t *pairs;
+  uint64_t coin_seed, coin0, coin_per_call;
   uint32_t keys[EA_MAX_REPLICAS];
 };
 
-struct PtArgs {
-  ea_hq_state *st;       // [ns][nr]
-  int32_t *ener;         // [ns][nr][nt] by clone
-  const int32_t *part;   // sweep partials [ns][nr][nt][nbx], or NULL: use ener
-  int nbx;
-  uint8_t *wctb;         // [ns][nr][nt]
-  double *acc, *att, *raw;  // [ns][nr][nt-1]
-  const double *nd;      // [nt-1]  -(beta[i+1]-beta[i])
-  const double *tab;     // exp tables (host glibc values)
-  const int32_t *off, *len;
-  double *esum;          // per-beta energy sums, added after the swaps
-  uint8_t *hist;         // beta-of-clone history row, when row >= 0
-  int measure, row;
+// Position of a launch inside a step, fixed when the step is enqueued.
+struct Phase {
+  int half;     // half-sweep index within the step (sweep), parity = half & 1
+  int energy;   // sweep: write the energy partials
+  int call;     // Houdayer call index within the step
+  int measure;  // PT / Houdayer: accumulate esum and history
 };
 
-struct HouArgs {
-  uint64_t *spin;
-  const uint64_t *J;     // [ns][2][n][6]
-  int32_t *ener;         // [ns][nr][nt] by clone; updated by the whole-tube path
-  const uint8_t *wctb;
-  int32_t pairs[4];
-  uint64_t seed, offset0;  // coin index of (group 0, sample 0, beta 0, site 0)
-  uint32_t *lab;         // [2][ns][ctm][V] global labels (tiled path)
-  uint32_t *cbits;       // [2][ns][ctm][V/32] root coins (tiled path)
-  uint32_t *labels_out;  // optional copy of the final labels
-  int planes;            // z-planes per slab; 0: the whole tube in one block
-  double *esum;          // whole-tube path: per-beta sums corrected for the
-  int measure;           // moved configurations when measure != 0
+constexpr int kMapShift = 20;  // block map entry: tube << 20 | local block
+
+// Block maps of every kernel kind over one table of tubes.
+// kMapEnergyU / kMapChainU cover only the tubes whose step measures with a
+// separate energy pass (cluster moves on the slab path).
+enum MapKind {
+  kMapSweep, kMapEnergy, kMapChain, kMapWhole32, kMapWhole64, kMapSlab,
+  kMapSeam, kMapFlatten, kMapApply, kMapEnergyU, kMapChainU, kMapKinds
 };
 
-struct MeasureArgs {
-  const int32_t *ener;
-  const uint8_t *wctb;
-  double *esum;          // [ns][nr][nt] by beta
-  uint8_t *hist;         // [rows][ns][nr][nt] beta of clone
-  int row;               // history row to write, < 0 for none
+// A tube's step measures its energies inside PT (and the whole-tube cluster
+// move corrects them) unless its cluster moves take the slab path.
+inline __host__ __device__ bool measured_in_pt(const Tube &t) {
+  return !t.cluster || t.planes == 0;
+}
+struct Plan {
+  std::vector<uint32_t> map[kMapKinds];
+  size_t lds32 = 0, lds64 = 0, lds_slab = 0;
 };
 
-hipError_t launch_sweep(const Lattice &g, const SweepArgs &a, hipStream_t s);
-hipError_t launch_energy(const Lattice &g, const uint64_t *spin,
-                         const uint64_t *J, int32_t *ener, int32_t *part,
-                         hipStream_t s);
-// ener by clone from the partials the last energy-writing sweep left by beta
-// (wctb as it was during that sweep)
-hipError_t launch_resolve(const Lattice &g, const int32_t *part, const uint8_t *wctb,
-                          int32_t *ener, hipStream_t s);
-// int32 slots of the partial-energy buffer both kinds of partials need
-size_t energy_partials(const Lattice &g);
+// Device copy of a plan: one buffer holding all maps.
+struct DevPlan {
+  uint32_t *buf = nullptr;
+  size_t off[kMapKinds] = {};
+  size_t cnt[kMapKinds] = {};
+  size_t lds32 = 0, lds64 = 0, lds_slab = 0;
+  const uint32_t *map(int k) const { return buf + off[k]; }
+};
+
+// Sweep blocks per beta, and partial-energy slots a tube needs.
 int sweep_blocks_x(const Lattice &g);
-hipError_t launch_pt(const Lattice &g, const PtArgs &a, hipStream_t s);
-hipError_t launch_houdayer(const Lattice &g, const HouArgs &a, hipStream_t s);
-hipError_t launch_measure(const Lattice &g, const MeasureArgs &a,
-                          hipStream_t s);
+size_t energy_partials(const Lattice &g);
+
+// Block maps of the tubes (host copies of the descriptors give geometry,
+// Houdayer path and whether the tube has cluster moves).
+Plan make_plan(const std::vector<Tube> &tubes);
+hipError_t upload_plan(const Plan &p, DevPlan *d);
+void free_plan(DevPlan *d);
+
+// Launches over a device tube table.  Each returns hipGetLastError().
+hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
+hipError_t launch_energy(const Tube *tubes, const DevPlan &p, bool unfused_only, hipStream_t s);
+hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, hipStream_t s);
+hipError_t launch_pt(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
+hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
+hipError_t launch_measure(const Tube *tubes, const DevPlan &p, bool unfused_only, hipStream_t s);
+hipError_t launch_step_end(const Tube *tubes, int ntubes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
 

@@ -64,7 +64,7 @@ EXPORTS = [
     "ea_hq_init", "ea_hq_next", "ea_seeds_from_one", "ea_init_generators",
     "ea_fill_counters", "ea_make_lut", "ea_init_sample",
     "ea_houdayer_pairs", "ea_unpack_lattice", "ea_set_timing",
-    "ea_read_timing",
+    "ea_read_timing", "ea_batch_create", "ea_batch_destroy", "ea_batch_run",
 ]
 KINDS = ("sweep", "energy", "pt", "houdayer", "measure")
 
@@ -111,6 +111,9 @@ def lib():
                                        C.c_int]),
         "ea_set_timing": (C.c_int, [vp, C.c_int]),
         "ea_read_timing": (C.c_int, [vp, p(C.c_double), p(C.c_int64)]),
+        "ea_batch_create": (C.c_int, [p(vp), C.c_int, p(vp)]),
+        "ea_batch_destroy": (C.c_int, [vp]),
+        "ea_batch_run": (C.c_int, [vp, p(Schedule), p(u64)]),
         "ea_hq_init": (None, [p(HQ), u64]),
         "ea_hq_next": (u64, [p(HQ)]),
         "ea_seeds_from_one": (None, [u64, p(u64), C.c_int]),
@@ -338,6 +341,35 @@ class Engine:
         check(lib().ea_read_measures(self.h, _ptr(es, C.c_double),
                                      _ptr(hist, C.c_uint8), rows))
         return es, hist[:rows]
+
+
+class Batch:
+    """Several engines' tubes advancing in the same launches: one step of
+    every tube captured in a HIP graph and replayed (ea_batch_run)."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        self.h = C.c_void_p()
+        check(lib().ea_batch_create(arr, len(self.engines), C.byref(self.h)))
+
+    def run(self, schedules):
+        """schedules[i] for engine i; returns the coin offsets after the run."""
+        arr = (Schedule * len(schedules))(*schedules)
+        off = np.zeros(len(schedules), np.uint64)
+        check(lib().ea_batch_run(self.h, arr, _ptr(off, C.c_uint64)))
+        return off
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().ea_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------- simulation

@@ -203,6 +203,27 @@ enum {
 int ea_set_timing(ea_engine *e, int on);
 int ea_read_timing(ea_engine *e, double *ms, int64_t *launches);
 
+/* ----------------------------------------------------------------- batch */
+
+/* No counterpart in the reference, which runs one tube size per process
+ * (PROGRAM/Makefile:3-36): a batch advances several engines' tubes (any
+ * mix of L, Lz, boundary condition, NT and samples on one device) in the
+ * same kernel launches.  One step of every tube (sweeps, PT, Houdayer and
+ * the energy measures, as ea_run) is captured once in a HIP graph and the
+ * graph is replayed; the arithmetic is the engines' own, so each tube ends
+ * bit-identical to ea_run on that engine alone.  The engines stay usable on
+ * their own between batch runs. */
+typedef struct ea_batch ea_batch;
+int ea_batch_create(ea_engine *const *engines, int n, ea_batch **out);
+int ea_batch_destroy(ea_batch *b);
+
+/* ea_run for every engine of the batch, schedules[i] for engine i.  The
+ * schedules share mesfr, houfr and ptfr; n_it, cluster, it0, frec_trw and the
+ * inputs are per tube (a tube with fewer steps idles while the others go
+ * on).  Returns when all steps are done; rng_offsets_out[i] (may be NULL)
+ * receives engine i's coin offset after its last Houdayer call. */
+int ea_batch_run(ea_batch *b, const ea_schedule *schedules, uint64_t *rng_offsets_out);
+
 /* ------------------------------------------------- host helpers (no GPU) */
 
 /* Init_Rand_HQ_64bits (random_generator.cu:160-201) and one draw

@@ -1,0 +1,110 @@
+"""The batched form: several tubes of different L, Lz, boundary condition,
+temperatures and samples advancing in the same launches (one step captured
+in a HIP graph and replayed) end bit-identical to each tube run alone, and
+to the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import eatube
+import orc
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = lambda k: (1000 + 4 * k, 1001 + 4 * k, 1002 + 4 * k, 1003 + 4 * k)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(spec, k):
+    L, Lz, bc, nt, ns = spec
+    betas = eatube.betas_linear(0.6, 1.6, nt)
+    return eatube.Simulation(L, Lz, 4, nt, ns, bc=bc, betas=betas,
+                             seeds=SEEDS(k), job_id=k)
+
+
+def state(sim):
+    e = sim.engine
+    st, acc, att, raw = e.download_pt_state()
+    es, hist = e.read_measures(rows=4)
+    return dict(spins=sim.spins(), perms=e.download_perms(), pt=bytes(st),
+                acc=acc, att=att, raw=raw, esum=es, hist=hist,
+                ener=e.energy())
+
+
+def same(a, b, what):
+    for k in a:
+        if isinstance(a[k], bytes):
+            assert a[k] == b[k], "%s: %s differ" % (what, k)
+        else:
+            assert np.array_equal(a[k], b[k]), "%s: %s differ" % (what, k)
+
+
+MIX = [
+    (8, 16, 0, 6, 2),     # small PBC
+    (8, 64, 1, 8, 1),     # OBC, Lz = 8L
+    (16, 16, 1, 5, 2),    # OBC cube
+    (16, 64, 0, 4, 1),    # PBC, Lz = 4L
+    (24, 24, 0, 4, 1),    # L = 24
+    (24, 192, 1, 4, 1),   # slab-path Houdayer (labels do not fit in LDS)
+]
+
+
+def test_batch_equals_tube_by_tube():
+    plain = [make(s, k) for k, s in enumerate(MIX)]
+    twins = [make(s, k) for k, s in enumerate(MIX)]
+    say("tubes", MIX, "slab planes", [t.engine.ctm for t in twins])
+    batch = eatube.Batch([t.engine for t in twins])
+    for rnd, n_it in enumerate((3, 2)):
+        scheds = [t.schedule(n_it, cluster=True, it0=rnd * 3, frec_trw=2)
+                  for t in twins]
+        offs = batch.run(scheds)
+        for k, p in enumerate(plain):
+            sch = p.schedule(n_it, cluster=True, it0=rnd * 3, frec_trw=2)
+            p.engine.run(sch)
+            assert offs[k] == p.rng_offset, "coin offset of tube %d" % k
+        for k, (p, t) in enumerate(zip(plain, twins)):
+            say("round", rnd, "tube", k, MIX[k])
+            same(state(p), state(t), "round %d tube %d %s" % (rnd, k, MIX[k]))
+    batch.close()
+
+
+def test_batch_uneven_steps_and_no_cluster_tubes():
+    """Tubes with different step counts, some without cluster moves."""
+    specs = [(8, 16, 0, 6, 2), (16, 32, 1, 6, 1), (8, 32, 1, 4, 3)]
+    plain = [make(s, 10 + k) for k, s in enumerate(specs)]
+    twins = [make(s, 10 + k) for k, s in enumerate(specs)]
+    batch = eatube.Batch([t.engine for t in twins])
+    n_its, clus = (4, 1, 3), (True, False, True)
+    scheds = [t.schedule(n, cluster=c) for t, n, c in zip(twins, n_its, clus)]
+    batch.run(scheds)
+    for p, n, c in zip(plain, n_its, clus):
+        p.steps(n, cluster=c)
+    for k, (p, t) in enumerate(zip(plain, twins)):
+        same(state(p), state(t), "tube %d %s" % (k, specs[k]))
+    batch.close()
+
+
+def test_batch_matches_oracle():
+    specs = [(8, 16, 1, 6, 2), (16, 16, 0, 4, 1)]
+    sims = [make(s, 20 + k) for k, s in enumerate(specs)]
+    oracles = []
+    for k, (L, Lz, bc, nt, ns) in enumerate(specs):
+        oracles.append(orc.Sim(L, Lz, 4, nt, ns, bc=bc,
+                               betas=orc.betas_linear(0.6, 1.6, nt),
+                               seeds=SEEDS(20 + k), job_id=20 + k))
+    batch = eatube.Batch([s.engine for s in sims])
+    batch.run([s.schedule(2, cluster=True) for s in sims])
+    for o in oracles:
+        o.steps(2, cluster=True)
+    for k, (s, o) in enumerate(zip(sims, oracles)):
+        assert np.array_equal(s.spins(), o.spins()), "tube %d spins" % k
+        assert np.array_equal(s.engine.download_perms().ravel(),
+                              o.field("wctb")), "tube %d perms" % k
+        es, _ = s.engine.read_measures()
+        assert np.array_equal(es.ravel(), o.field("esum")), "tube %d esum" % k
+    batch.close()
