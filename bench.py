@@ -92,6 +92,91 @@ def parity_check(w):
     return ok
 
 
+def mixed_tubes(n=48, seed=7):
+    """The uneven mix: n tubes, L from {8, 16, 24} (multiples of 8 only:
+    MSC needs L/4 even), Lz from {L, 4L, 8L}, half PBC and half OBC, 4
+    replicas, 24 temperatures in [0.6, 1.6], per-tube seeds."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        L = int(rng.choice([8, 16, 24]))
+        Lz = L * int(rng.choice([1, 4, 8]))
+        out.append((L, Lz, i % 2))
+    return out
+
+
+def batched_compare(steps, samples=1, ntubes=48):
+    """The same tubes from the same seeds, advanced `steps` steps (sweep +
+    PT + Houdayer + measure) tube by tube with ea_run, and as one batch
+    (ea_batch_run); states must end bit-identical."""
+    tubes = mixed_tubes(ntubes)
+    nt, nr = 24, 4
+    betas = eatube.betas_linear(0.6, 1.6, nt)
+    lut = eatube.make_luts(betas)
+
+    def sims():
+        return [eatube.Simulation(L, Lz, nr, nt, samples, bc=bc, betas=betas,
+                                  lut=lut, seeds=(9000 + 4 * k, 9001 + 4 * k,
+                                                  9002 + 4 * k, 9003 + 4 * k),
+                                  job_id=k)
+                for k, (L, Lz, bc) in enumerate(tubes)]
+    plain, twins = sims(), sims()
+    attempts = steps * sum(L * L * Lz for L, Lz, _ in tubes) * nr * nt * samples
+    # plain form: one tube after the other, kernels launched per tube
+    for p in plain:  # warm-up call (module load, plans)
+        p.steps(1, cluster=True)
+        p.engine.sync()
+    scheds = [p.schedule(steps, cluster=True) for p in plain]
+    t0 = time.perf_counter()
+    for p, sch in zip(plain, scheds):
+        p.engine.run(sch)
+    for p in plain:
+        p.engine.sync()
+    t_plain = time.perf_counter() - t0
+    # batched form
+    batch = eatube.Batch([t.engine for t in twins])
+    batch.run([t.schedule(1, cluster=True) for t in twins])
+    scheds = [t.schedule(steps, cluster=True) for t in twins]
+    t0 = time.perf_counter()
+    batch.run(scheds)
+    t_batch = time.perf_counter() - t0
+    same = True
+    for p, t in zip(plain, twins):
+        same = same and np.array_equal(p.spins(), t.spins()) and \
+            np.array_equal(p.engine.download_perms(), t.engine.download_perms())
+    batch.close()
+    for x in plain + twins:
+        x.engine.close()
+    return {"tubes": len(tubes), "steps": steps, "samples_per_tube": samples,
+            "sites": int(sum(L * L * Lz for L, Lz, _ in tubes)),
+            "plain_value": attempts / (t_plain * 1e9),
+            "batched_value": attempts / (t_batch * 1e9),
+            "speedup": t_plain / t_batch, "bit_exact": bool(same),
+            "unit": "spin-flip attempts/ns"}
+
+
+def serving(nreq, rate, sweeps, budget_gb=16.0, chunk=32):
+    """Poisson stream of simulation requests through the continuous-batching
+    server: p50/p99 completion latency (arrival to completion) and
+    whole-run throughput."""
+    import serve
+    reqs = serve.poisson_requests(nreq, rate, seed=5, sweeps=sweeps)
+    srv = serve.Server(budget=int(budget_gb * 2 ** 30), chunk=chunk)
+    t0 = time.perf_counter()
+    res = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    out = serve.summary(reqs, res, wall)
+    out.update({"rate_per_s": rate, "sweeps_range": list(sweeps),
+                "chunk_steps": chunk, "hbm_budget_gb": budget_gb,
+                "max_running": srv.stats["max_running"],
+                "batches_built": srv.stats["batches_built"],
+                "request_shape": "L in {8,16,24,32}, Lz multiple of 8 in "
+                                 "[L,16L], PBC|OBC, NT in {16,24,32} in "
+                                 "[0.6,1.6], 4 replicas, 1 sample, sweep+PT "
+                                 "per step"})
+    return out
+
+
 def cpu_baseline(w, S, budget_s):
     """The oracle (scalar C restatement of the reference algorithm) on one
     host core, over a bounded number of steps of the same workload."""
@@ -125,6 +210,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=50)
+    ap.add_argument("--batch-steps", type=int, default=200,
+                    help="steps of the 48-tube mixed batch comparison (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=40,
+                    help="requests of the serving run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU; runs on %d GPUs are not part of "
@@ -216,6 +306,13 @@ def main():
         out["sweep_roofline"] = {"achieved": sweep["GBs"], "peak": HBM_PEAK_GBS,
                                  "frac": sweep["GBs"] / HBM_PEAK_GBS,
                                  "avg_launch_us": sweep["avg_us"]}
+    if args.batch_steps > 0:
+        log("batched vs plain, 48 mixed tubes")
+        out["batched"] = batched_compare(args.batch_steps)
+    if args.serve_requests > 0:
+        log("serving %d requests" % args.serve_requests)
+        out["serving"] = serving(args.serve_requests, args.serve_rate,
+                                 (200, 2000))
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(w, S, args.cpu_seconds)

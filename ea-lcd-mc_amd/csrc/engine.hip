@@ -367,6 +367,40 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   return EA_OK;
 }
 
+// Device bytes ea_create allocates for cfg, with lds_max bytes of LDS per
+// workgroup deciding the Houdayer path.
+static uint64_t engine_bytes(const ea_config *cfg, int lds_max) {
+  Lattice g{};
+  g.L = cfg->L;
+  g.Lz = cfg->Lz;
+  g.V = cfg->L * cfg->L * cfg->Lz;
+  g.n = g.V / 128;
+  g.nr = cfg->nr;
+  g.nt = cfg->nt;
+  g.ns = cfg->ns;
+  g.ctm = cfg->clone_threshold;
+  g.half = (int64_t)g.n * g.nr * g.nt;
+  const uint64_t chains = (uint64_t)g.ns * g.nr;
+  uint64_t b = 8ull * g.ns * 2 * g.half + 8ull * g.ns * 12 * g.n + chains * g.nt +
+               (uint64_t)g.nt * EA_LUT_BYTES + 4ull * chains * g.nt + sizeof(ea_hq_state) * chains +
+               4ull * energy_partials(g) + 24ull * chains * (g.nt - 1) + 8ull * chains * g.nt +
+               sizeof(int) + sizeof(Tube);
+  const ExpTables x = make_exp_tables(cfg->betas, g.nt, g.V);
+  b += 8ull * x.nd.size() + 8ull * x.off.size() + 8ull * std::max<size_t>(x.val.size(), 1);
+  if (g.ctm > 0 && hou_whole_lds(g) > (size_t)lds_max) {
+    const uint64_t inst = 2ull * g.ns * g.ctm;
+    b += 4ull * inst * g.V + 4ull * inst * (g.V / 32);
+  }
+  return b;
+}
+
+extern "C" int ea_footprint(const ea_config *cfg, uint64_t *bytes) {
+  if (!bytes) return fail(EA_E_ARG, "bytes is NULL");
+  RC_TRY(check_config(cfg));
+  *bytes = engine_bytes(cfg, 160 * 1024);
+  return EA_OK;
+}
+
 extern "C" int ea_destroy(ea_engine *e) {
   if (!e) return EA_OK;
   (void)hipStreamSynchronize(e->stream);
@@ -829,7 +863,7 @@ extern "C" int ea_batch_run(ea_batch *b, const ea_schedule *sch, uint64_t *rng_o
   }
   HIP_TRY(hipStreamSynchronize(b->stream));
   HIP_TRY(hipMemcpy(b->dt, tubes.data(), sizeof(Tube) * n, hipMemcpyHostToDevice));
-  if (cluster != b->plan_cluster || !b->plan.buf) {
+  if (cluster != b->plan_cluster || !b->plan.ready()) {
     RC_TRY(drop_graph(b));
     HIP_TRY(upload_plan(make_plan(tubes), &b->plan));
     b->plan_cluster = cluster;

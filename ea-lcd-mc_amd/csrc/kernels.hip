@@ -96,13 +96,14 @@ __device__ __forceinline__ uint32_t qdiv(uint32_t i, uint32_t m) {
 }
 
 // The tube of this block and the block's index within the tube (block map
-// entry tube << kMapShift | local); the tube's step counter is read once.
+// entry tube << kMapShift | local; no map: tube 0, local = block index); the
+// tube's step counter is read once.
 struct Where {
   const Tube *t;
   int local, step;
 };
 __device__ __forceinline__ Where locate(const Tube *tubes, const uint32_t *map) {
-  const uint32_t e = map[blockIdx.x];
+  const uint32_t e = map ? map[blockIdx.x] : blockIdx.x;
   Where w;
   w.t = tubes + (e >> kMapShift);
   w.local = (int)(e & ((1u << kMapShift) - 1));
@@ -1164,6 +1165,7 @@ size_t hou_slab_lds(const Lattice &g, int planes) {
 
 Plan make_plan(const std::vector<Tube> &tubes) {
   Plan p;
+  p.ntubes = (int)tubes.size();
   for (size_t ti = 0; ti < tubes.size(); ++ti) {
     const Tube &t = tubes[ti];
     const Lattice &g = t.g;
@@ -1211,7 +1213,8 @@ hipError_t upload_plan(const Plan &p, DevPlan *d) {
   d->lds32 = p.lds32;
   d->lds64 = p.lds64;
   d->lds_slab = p.lds_slab;
-  if (!total) return hipSuccess;
+  d->identity = p.ntubes == 1;
+  if (!total || d->identity) return hipSuccess;
   hipError_t e = hipMalloc(reinterpret_cast<void **>(&d->buf), total * sizeof(uint32_t));
   if (e != hipSuccess) return e;
   for (int k = 0; k < kMapKinds; ++k)

@@ -102,15 +102,20 @@ inline __host__ __device__ bool measured_in_pt(const Tube &t) {
 struct Plan {
   std::vector<uint32_t> map[kMapKinds];
   size_t lds32 = 0, lds64 = 0, lds_slab = 0;
+  int ntubes = 0;
 };
 
-// Device copy of a plan: one buffer holding all maps.
+// Device copy of a plan: one buffer holding all maps.  The maps of a
+// one-tube table are the identity (entry i = i) and are not stored: map()
+// returns NULL and a block's local index is its block index.
 struct DevPlan {
   uint32_t *buf = nullptr;
   size_t off[kMapKinds] = {};
   size_t cnt[kMapKinds] = {};
   size_t lds32 = 0, lds64 = 0, lds_slab = 0;
-  const uint32_t *map(int k) const { return buf + off[k]; }
+  bool identity = false;
+  const uint32_t *map(int k) const { return identity ? nullptr : buf + off[k]; }
+  bool ready() const { return identity || buf; }
 };
 
 // Sweep blocks per beta, and partial-energy slots a tube needs.

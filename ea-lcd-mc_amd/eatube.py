@@ -65,6 +65,7 @@ EXPORTS = [
     "ea_fill_counters", "ea_make_lut", "ea_init_sample",
     "ea_houdayer_pairs", "ea_unpack_lattice", "ea_set_timing",
     "ea_read_timing", "ea_batch_create", "ea_batch_destroy", "ea_batch_run",
+    "ea_footprint",
 ]
 KINDS = ("sweep", "energy", "pt", "houdayer", "measure")
 
@@ -114,6 +115,7 @@ def lib():
         "ea_batch_create": (C.c_int, [p(vp), C.c_int, p(vp)]),
         "ea_batch_destroy": (C.c_int, [vp]),
         "ea_batch_run": (C.c_int, [vp, p(Schedule), p(u64)]),
+        "ea_footprint": (C.c_int, [p(Config), p(u64)]),
         "ea_hq_init": (None, [p(HQ), u64]),
         "ea_hq_next": (u64, [p(HQ)]),
         "ea_seeds_from_one": (None, [u64, p(u64), C.c_int]),
@@ -197,6 +199,18 @@ def init_sample(L, Lz, nr, nt, bc, variant, mattis, seed_J, seed_u):
     if cnt < 0:
         check(cnt)
     return ev, od, J, cp, cnt
+
+
+def footprint(L, Lz, nr, nt, ns, bc, betas, clone_thr=None):
+    """Device bytes an engine of this configuration allocates."""
+    betas = np.ascontiguousarray(betas, np.float64)
+    lut = np.zeros(1, np.uint8)  # not read
+    ctm = clone_threshold(nt) if clone_thr is None else clone_thr
+    c = Config(L, Lz, nr, nt, ns, bc, ctm, 0, _ptr(betas, C.c_double),
+               _ptr(lut, C.c_uint8))
+    out = C.c_uint64()
+    check(lib().ea_footprint(C.byref(c), C.byref(out)))
+    return out.value
 
 
 def unpack_lattice(L, Lz, nr, nt, even, odd, r, c):

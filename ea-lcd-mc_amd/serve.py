@@ -1,0 +1,211 @@
+"""Serving loop for EA tube simulations on one GPU: continuous batching.
+
+Requests (seed, L, Lz, boundary condition, temperatures, sweeps) arrive
+over time.  The server admits a request as soon as it has arrived and its
+device footprint (ea_footprint) fits the HBM budget next to the running
+ones; its lattice, couplings and generator states then stay on the device
+until it completes.  All running requests advance together: the engines'
+tubes, ordered by size, form one batch (ea_batch_run, one step of every tube
+per graph replay) that runs in chunks of a few steps; between chunks,
+finished requests leave (their memory is freed) and newly arrived ones join,
+so requests that arrive while others are on the GPU overlap with them.  A
+request's steps are the same whether it runs alone or in the mix: its
+counter blocks, replica pairs and coins come from its own host streams, and
+the chunks of a run are its consecutive steps.
+
+The reference has no counterpart (it runs one tube size per process,
+PROGRAM/Makefile:3-36); each request is that program's run of one job.
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import eatube
+
+
+@dataclass
+class Request:
+    rid: int
+    seed: int
+    L: int
+    Lz: int
+    bc: int
+    nt: int
+    sweeps: int
+    arrival: float = 0.0  # seconds after the start of the run
+    nr: int = 4
+    ns: int = 1
+    tmin: float = 0.6
+    tmax: float = 1.6
+    cluster: bool = False
+
+    def seeds(self):
+        s = int(self.seed) * 4
+        return (s + 1, s + 2, s + 3, s + 4)
+
+    def attempts(self):
+        return self.sweeps * self.L * self.L * self.Lz * self.nr * self.nt * self.ns
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float
+    finished: float
+    spins: np.ndarray = None
+    perms: np.ndarray = None
+    esum: np.ndarray = None
+    energies: np.ndarray = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def poisson_requests(n, rate, seed=1, sizes=(8, 16, 24, 32), nts=(16, 24, 32),
+                     sweeps=(1000, 100000), lz_max=16):
+    """n requests at Poisson arrivals of `rate` per second: L from `sizes`,
+    Lz a multiple of 8 in [L, lz_max*L], PBC or OBC, NT from `nts`, sweeps
+    log-uniform in [sweeps[0], sweeps[1]]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        L = int(rng.choice(sizes))
+        Lz = 8 * int(rng.integers(L // 8, lz_max * L // 8 + 1))
+        sw = int(round(math.exp(rng.uniform(math.log(sweeps[0]),
+                                            math.log(sweeps[1])))))
+        out.append(Request(rid=i, seed=1 + 7919 * i, L=L, Lz=Lz,
+                           bc=int(rng.integers(0, 2)), nt=int(rng.choice(nts)),
+                           sweeps=sw, arrival=t))
+    return out
+
+
+class _Luts:
+    """LUTs by (nt, tmin, tmax): built once (quad precision on the host)."""
+
+    def __init__(self):
+        self.cache = {}
+
+    def get(self, r):
+        key = (r.nt, r.tmin, r.tmax)
+        if key not in self.cache:
+            betas = eatube.betas_linear(r.tmin, r.tmax, r.nt)
+            self.cache[key] = (betas, eatube.make_luts(betas))
+        return self.cache[key]
+
+
+def start(r, luts):
+    """Device-resident state of request r, initialised as the reference
+    program does (eatube.Simulation)."""
+    betas, lut = luts.get(r)
+    return eatube.Simulation(r.L, r.Lz, r.nr, r.nt, r.ns, bc=r.bc, betas=betas,
+                             lut=lut, seeds=r.seeds(), job_id=r.rid % (1 << 14))
+
+
+def footprint(r, luts):
+    betas, _ = luts.get(r)
+    return eatube.footprint(r.L, r.Lz, r.nr, r.nt, r.ns, r.bc, betas)
+
+
+class Server:
+    """Continuous batching of requests within `budget` bytes of HBM.
+
+    chunk      steps every running request advances between admissions
+    max_tubes  most requests on the GPU at once
+    keep       keep each result's final state (spins, perms, energy sums)
+    """
+
+    def __init__(self, budget, chunk=32, max_tubes=256, keep=False,
+                 start_fn=start, batch_fn=eatube.Batch, footprint_fn=footprint,
+                 clock=time.perf_counter):
+        self.budget = int(budget)
+        self.chunk = int(chunk)
+        self.max_tubes = int(max_tubes)
+        self.keep = keep
+        self.start_fn, self.batch_fn = start_fn, batch_fn
+        self.footprint_fn, self.clock = footprint_fn, clock
+        self.luts = _Luts()
+        self.stats = dict(chunks=0, batches_built=0, max_running=0,
+                          max_bytes=0, deferred=0)
+
+    def run(self, requests):
+        """Serve `requests` (arrival times relative to now); returns the
+        results in request order."""
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        t0 = self.clock()
+        running = {}  # rid -> [request, sim, steps left, bytes, admitted]
+        used = 0
+        results = {}
+        batch, members = None, ()
+        while pending or running:
+            now = self.clock() - t0
+            # admission: arrived requests, in arrival order, while they fit
+            while pending and pending[0].arrival <= now and \
+                    len(running) < self.max_tubes:
+                r = pending[0]
+                need = self.footprint_fn(r, self.luts)
+                if need > self.budget:
+                    raise ValueError("request %d needs %d bytes, more than the "
+                                     "budget" % (r.rid, need))
+                if used + need > self.budget:
+                    self.stats["deferred"] += 1
+                    break
+                pending.pop(0)
+                running[r.rid] = [r, self.start_fn(r, self.luts), r.sweeps,
+                                  need, self.clock() - t0]
+                used += need
+            self.stats["max_running"] = max(self.stats["max_running"], len(running))
+            self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
+            if not running:
+                time.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
+                continue
+            # the running requests, ordered by size, are the batch's tubes
+            order = tuple(sorted(running, key=lambda k: (
+                running[k][0].L * running[k][0].L * running[k][0].Lz, k)))
+            if order != members:
+                if batch is not None:
+                    batch.close()
+                batch = self.batch_fn([running[k][1].engine for k in order])
+                members = order
+                self.stats["batches_built"] += 1
+            scheds = []
+            for k in order:
+                r, sim, left = running[k][:3]
+                scheds.append(sim.schedule(min(self.chunk, left),
+                                           cluster=r.cluster))
+            batch.run(scheds)
+            self.stats["chunks"] += 1
+            done = self.clock() - t0
+            for k, sch in zip(order, scheds):
+                running[k][2] -= sch.n_it
+                if running[k][2] == 0:
+                    r, sim, _, need, adm = running.pop(k)
+                    res = Result(r.rid, r.arrival, adm, done)
+                    if self.keep:
+                        es, _ = sim.engine.read_measures()
+                        res.spins = sim.spins()
+                        res.perms = sim.engine.download_perms()
+                        res.esum = es
+                        res.energies = sim.engine.energy()
+                    sim.engine.close()
+                    used -= need
+                    results[r.rid] = res
+        if batch is not None:
+            batch.close()
+        return [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+
+
+def summary(requests, results, wall):
+    lat = np.array([x.latency for x in results])
+    work = sum(r.attempts() for r in requests)
+    return {"requests": len(results),
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()),
+            "wall_s": wall,
+            "attempts_per_ns": work / (wall * 1e9)}

@@ -102,6 +102,11 @@ int ea_device_count(int *count);
  * Binds the calling thread's current HIP device. */
 int ea_create(const ea_config *cfg, ea_engine **out);
 
+/* Device bytes ea_create(cfg) allocates on gfx950 (160 KiB of LDS per
+ * workgroup), before the per-call inputs and the TRW history; no GPU needed.
+ * No counterpart in the reference: admission by HBM footprint. */
+int ea_footprint(const ea_config *cfg, uint64_t *bytes);
+
 /* Free everything: FreeDevice (main.cu:748-768), SWclose (HoudayerXEA.cu:214). */
 int ea_destroy(ea_engine *e);
 
