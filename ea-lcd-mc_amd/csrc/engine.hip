@@ -160,6 +160,14 @@ extern "C" int ea_device_count(int *count) {
   return EA_OK;
 }
 
+extern "C" int ea_select_device(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+    return fail(EA_E_NODEV, "device=%d count_devices=%d", device, n);
+  HIP_TRY(hipSetDevice(device));
+  return EA_OK;
+}
+
 static int check_config(const ea_config *c) {
   if (!c) return fail(EA_E_ARG, "config is NULL");
   if (!valid_lattice(c->L, c->Lz))

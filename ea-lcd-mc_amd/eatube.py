@@ -65,7 +65,8 @@ EXPORTS = [
     "ea_fill_counters", "ea_make_lut", "ea_init_sample",
     "ea_houdayer_pairs", "ea_unpack_lattice", "ea_set_timing",
     "ea_read_timing", "ea_batch_create", "ea_batch_destroy", "ea_batch_run",
-    "ea_footprint",
+    "ea_footprint", "ea_select_device", "ea_pack_lattice", "ea_conf_bytes",
+    "ea_conf_encode", "ea_conf_decode",
 ]
 KINDS = ("sweep", "energy", "pt", "houdayer", "measure")
 
@@ -116,6 +117,10 @@ def lib():
         "ea_batch_destroy": (C.c_int, [vp]),
         "ea_batch_run": (C.c_int, [vp, p(Schedule), p(u64)]),
         "ea_footprint": (C.c_int, [p(Config), p(u64)]),
+        "ea_select_device": (C.c_int, [C.c_int]),
+        "ea_pack_lattice": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int,
+                                      p(C.c_uint8), C.c_int, C.c_int, p(u64),
+                                      p(u64)]),
         "ea_hq_init": (None, [p(HQ), u64]),
         "ea_hq_next": (u64, [p(HQ)]),
         "ea_seeds_from_one": (None, [u64, p(u64), C.c_int]),

@@ -97,6 +97,10 @@ const char *ea_last_error(void);
 /* Number of visible HIP devices (main.cu:64-66). */
 int ea_device_count(int *count);
 
+/* Make `device` the calling thread's current device: cudaSetDevice
+ * (main.cu:68-70).  Engines bind the device current at ea_create. */
+int ea_select_device(int device);
+
 /* Allocate device state and upload constants: replaces CopyConstOnDevice /
  * CopyDataOnDevice (main.cu:634-725) and ClusterInit (HoudayerXEA.cu:123-181).
  * Binds the calling thread's current HIP device. */
@@ -267,10 +271,42 @@ int ea_init_sample(int L, int Lz, int nr, int nt, int bc, int obc_variant,
 int ea_houdayer_pairs(ea_hq_state *cl, int nr, int32_t pairs[4]);
 
 /* Unpack / pack one (replica, clone) lattice between the MSC layout and
- * one byte per site (unpacking_u / packing_u, ini.c:339-433). */
+ * one byte per site (unpacking_u / packing_u, ini.c:339-433).  Packing
+ * changes only that (replica, clone)'s bits. */
 int ea_unpack_lattice(int L, int Lz, int nr, int nt, const uint64_t *even,
                       const uint64_t *odd, int replica, int clone,
                       uint8_t *sites);
+int ea_pack_lattice(int L, int Lz, int nr, int nt, const uint8_t *sites,
+                    int replica, int clone, uint64_t *even, uint64_t *odd);
+
+/* s_data (header.h:91-106): run parameters in input.in order and master
+ * seeds, 72 bytes as conf files store them (pad_ is written as 0). */
+typedef struct {
+  int32_t nbin, itcut, itmax, houfr, mesfr, ptfr, nbetas, flag, l;
+  int32_t pad_;
+  uint64_t seed_J, seed_u, seed_MC, seed_Cluster;
+} ea_run_data;
+
+/* The checkpoint record of one (sample, replica): write_conf / read_conf
+ * (io.c:17-237).  ea_conf_bytes is its size (read_conf's predicted_size).
+ * Encode takes the sample's even/odd arrays (n*nr*nt words each) and the
+ * energies by clone; decode checks the size and the permutation, returns
+ * the energies by beta, and packs the spins into even/odd for `replica`
+ * (other replicas' bits are left as they are).  Output pointers of decode
+ * may be NULL. */
+size_t ea_conf_bytes(int L, int Lz, int nt);
+int ea_conf_encode(const ea_run_data *d, int L, int Lz, int nr, int nt,
+                   const double *betas, const ea_hq_state *pt,
+                   const ea_hq_state *prc, const ea_hq_state *cl,
+                   const ea_xo_state *xo, uint64_t coin_offset,
+                   const uint8_t *wctb, const uint64_t *even,
+                   const uint64_t *odd, int replica,
+                   const int32_t *ener_by_clone, uint8_t *out);
+int ea_conf_decode(const uint8_t *buf, size_t len, int L, int Lz, int nr,
+                   int nt, ea_run_data *d, double *betas, ea_hq_state *pt,
+                   ea_hq_state *prc, ea_hq_state *cl, ea_xo_state *xo,
+                   uint64_t *coin_offset, uint8_t *wctb, uint64_t *even,
+                   uint64_t *odd, int replica, int32_t *ener_by_beta);
 
 #ifdef __cplusplus
 }

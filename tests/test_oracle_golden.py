@@ -120,15 +120,14 @@ def test_fill_bits(golden):
     golden.check("fill_blocks", out)
 
 
-def test_parallel_tempering(golden):
+def replay_pt(golden, s, nsteps=25):
+    """Parallel_Tempering as the fixture harness ran it: nsteps steps from
+    the initial generators, with the scalar energies perturbed by the
+    harness's rule.  Returns the end state and the per-step records."""
     m = golden.meta
     nr, nt, ns = m["nr"], m["nt"], m["nbits"]
-    s = make_sim(golden)
     pts = (orc.HQ * (ns * nr)).from_buffer_copy(bytes(s.field("pt")))
     z = golden.z
-    nsteps = 25
-    # the energies the harness fed to Parallel_Tempering are reproduced from
-    # the (pinned) scalar energies with the harness's perturbation rule
     es = np.frombuffer(golden.raw("escalar"), np.int32) if golden.has(
         "escalar") and ("raw:escalar" in z) else None
     if es is None:
@@ -165,12 +164,20 @@ def test_parallel_tempering(golden):
                 wctb[i, r], wbtc[i, r] = w1, w2
                 acc[i, r], att[i, r], raw[i, r] = a1, a2, a3
         perms[st] = wctb
-    golden.check("pt_energies", energies)
-    golden.check("pt_perm", perms)
+    return dict(pts=pts, wctb=wctb, wbtc=wbtc, acc=acc, att=att, raw=raw,
+                perms=perms, energies=energies, escalar=es)
+
+
+def test_parallel_tempering(golden):
+    s = make_sim(golden)
+    r = replay_pt(golden, s)
+    acc, att, raw = r["acc"], r["att"], r["raw"]
+    golden.check("pt_energies", r["energies"])
+    golden.check("pt_perm", r["perms"])
     golden.check("pt_acc", np.concatenate([acc.ravel(), raw.ravel(),
                                            att.ravel()]))
-    golden.check("pt_final", bytes(pts))
-    golden.check("wbtc", wbtc)
+    golden.check("pt_final", bytes(r["pts"]))
+    golden.check("wbtc", r["wbtc"])
     # some swaps were accepted and some rejected
     assert 0 < raw.sum() < att.sum()
 
