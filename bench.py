@@ -75,6 +75,39 @@ def kernel_bytes(kind, w, S, V, ctm):
     return 0.0
 
 
+def copy_calibration(mib):
+    """A device-to-device copy of `mib` MiB (the runtime's blit kernel),
+    whose HBM traffic is known: mib MiB read and mib MiB written.  Profiled
+    beside the engine's kernels it converts FETCH_SIZE / WRITE_SIZE counter
+    readings into bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # device to device
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    profiles/traffic_<workload>.txt (two rocprofv3 --pmc runs, FETCH_SIZE
+    and WRITE_SIZE, scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def parity_check(w):
     """Two steps of two samples of the workload against the oracle."""
     import orc
@@ -215,6 +248,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=40,
                     help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a MIB-MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU; runs on %d GPUs are not part of "
@@ -225,6 +260,8 @@ def main():
     if eatube.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     w = WORKLOADS[args.workload]
     S = args.samples
     V = w["L"] * w["L"] * w["Lz"]
@@ -274,7 +311,7 @@ def main():
     ach = kern[dom]["GBs"]
     roof = {"bound": "hbm", "kernel": dom, "achieved": ach,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, dom),
             "bytes_per_launch": kernel_bytes(dom, w, S, V, ctm),
             "avg_launch_us": kern[dom]["avg_us"]}
     sweep = kern.get("sweep")

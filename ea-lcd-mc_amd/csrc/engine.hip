@@ -811,8 +811,6 @@ extern "C" int ea_read_timing(ea_engine *e, double *ms, int64_t *launches) {
 extern "C" int ea_batch_create(ea_engine *const *engines, int n, ea_batch **out) {
   if (!out || !engines || n < 1) return fail(EA_E_ARG, "invalid argument");
   *out = nullptr;
-  if (n > (1 << (32 - kMapShift)))
-    return fail(EA_E_ARG, "a batch holds at most %d tubes", 1 << (32 - kMapShift));
   for (int i = 0; i < n; ++i) {
     if (!engines[i]) return fail(EA_E_ARG, "engine %d is NULL", i);
     for (int j = 0; j < i; ++j)

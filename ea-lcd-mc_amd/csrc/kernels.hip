@@ -96,17 +96,17 @@ __device__ __forceinline__ uint32_t qdiv(uint32_t i, uint32_t m) {
 }
 
 // The tube of this block and the block's index within the tube (block map
-// entry tube << kMapShift | local; no map: tube 0, local = block index); the
-// tube's step counter is read once.
+// entry tube << 32 | local; no map: tube 0, local = block index); the tube's
+// step counter is read once.
 struct Where {
   const Tube *t;
   int local, step;
 };
-__device__ __forceinline__ Where locate(const Tube *tubes, const uint32_t *map) {
-  const uint32_t e = map ? map[blockIdx.x] : blockIdx.x;
+__device__ __forceinline__ Where locate(const Tube *tubes, const uint64_t *map) {
+  const uint64_t e = map ? map[blockIdx.x] : (uint64_t)blockIdx.x;
   Where w;
-  w.t = tubes + (e >> kMapShift);
-  w.local = (int)(e & ((1u << kMapShift) - 1));
+  w.t = tubes + (e >> 32);
+  w.local = (int)(uint32_t)e;
   w.step = *w.t->step;
   return w;
 }
@@ -215,7 +215,7 @@ constexpr int kSwPos = 64;                  // positions per block (one wave)
 constexpr int kSwSlots = 16;                // samples in flight per position
 constexpr int kSwThreads = kSwPos * kSwSlots;
 
-__global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *tubes, const uint32_t *map,
+__global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *tubes, const uint64_t *map,
                                                       Phase ph) {
   __shared__ uint32_t lut[EA_LUT_BYTES / 4];
   __shared__ uint8_t clone_at[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
@@ -323,7 +323,7 @@ __device__ __forceinline__ int energy_blocks_x(const Lattice &g) {
   return (g.n * g.nr + 255) / 256;
 }
 
-__global__ void __launch_bounds__(256) k_energy(const Tube *tubes, const uint32_t *map) {
+__global__ void __launch_bounds__(256) k_energy(const Tube *tubes, const uint64_t *map) {
   const Where wh = locate(tubes, map);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
@@ -382,7 +382,7 @@ __device__ __forceinline__ void sum_partials(const int32_t *part, int nt, int nw
 // ener[chain][c] from the chain's partials: the sweep's partials by beta
 // (BY_BETA, clone c = wctb[chain][b]) or k_energy's partials by clone.
 template <bool BY_BETA>
-__global__ void __launch_bounds__(64) k_energy_sum(const Tube *tubes, const uint32_t *map) {
+__global__ void __launch_bounds__(64) k_energy_sum(const Tube *tubes, const uint64_t *map) {
   __shared__ int32_t E[EA_MAX_BETAS];
   const Where wh = locate(tubes, map);
   const Tube &a = *wh.t;
@@ -419,7 +419,7 @@ __device__ __forceinline__ double exact_exp(const Tube &a, int i, int d) {
 // Both sides' errors are far below 1e-12, so every decision equals the
 // reference's.  The statistics are then added by one lane per pair with the
 // tabulated exp values.
-__global__ void __launch_bounds__(64) k_pt(const Tube *tubes, const uint32_t *map, Phase ph) {
+__global__ void __launch_bounds__(64) k_pt(const Tube *tubes, const uint64_t *map, Phase ph) {
   __shared__ uint64_t ira[256];
   __shared__ uint64_t zs[EA_MAX_BETAS], prs[EA_MAX_BETAS];
   __shared__ double uu[EA_MAX_BETAS], lu[EA_MAX_BETAS];
@@ -533,7 +533,7 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *tubes, const uint32_t *ma
 // --------------------------------------------------------------- measures
 
 // Per-beta energy sums and the beta-of-clone history (main.cu:208-228).
-__global__ void __launch_bounds__(64) k_measure(const Tube *tubes, const uint32_t *map) {
+__global__ void __launch_bounds__(64) k_measure(const Tube *tubes, const uint64_t *map) {
   const Where wh = locate(tubes, map);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
@@ -856,7 +856,7 @@ __device__ __forceinline__ uint64_t word_move_bits(const Lattice &g, const uint6
 // Dynamic LDS: rows[V/L] u64 | qw[2n] u64 | nw[2n] u64 | cb[V/32] u32 (coin
 // of every site) | forest[V] u32.
 template <typename RowT>
-__global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uint32_t *map,
+__global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uint64_t *map,
                                                     Phase ph) {
   using R = Rows<RowT>;
   extern __shared__ uint64_t lds[];
@@ -1036,7 +1036,7 @@ __device__ __forceinline__ int slab_count(const Tube &a) {
 }
 
 // Dynamic LDS: rows[planes*L] u64 | lab[planes*L*L] u32.
-__global__ void __launch_bounds__(1024) k_hou_slab(const Tube *tubes, const uint32_t *map,
+__global__ void __launch_bounds__(1024) k_hou_slab(const Tube *tubes, const uint64_t *map,
                                                    Phase ph) {
   extern __shared__ uint64_t lds[];
   const Where wh = locate(tubes, map);
@@ -1069,7 +1069,7 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *tubes, const uint
 
 // z bonds between plane z0-1 (or Lz-1 for the periodic wrap) and plane z0,
 // the first plane of the slab.
-__global__ void __launch_bounds__(256) k_hou_seam(const Tube *tubes, const uint32_t *map,
+__global__ void __launch_bounds__(256) k_hou_seam(const Tube *tubes, const uint64_t *map,
                                                   Phase ph) {
   const Where wh = locate(tubes, map);
   const Tube &a = *wh.t;
@@ -1094,7 +1094,7 @@ __global__ void __launch_bounds__(256) k_hou_seam(const Tube *tubes, const uint3
 __device__ __forceinline__ int site_blocks(const Lattice &g) { return (g.V + 255) / 256; }
 __device__ __forceinline__ int word_blocks(const Lattice &g) { return (2 * g.n + 255) / 256; }
 
-__global__ void __launch_bounds__(256) k_hou_flatten(const Tube *tubes, const uint32_t *map,
+__global__ void __launch_bounds__(256) k_hou_flatten(const Tube *tubes, const uint64_t *map,
                                                      Phase ph) {
   const Where wh = locate(tubes, map);
   const Tube &a = *wh.t;
@@ -1111,7 +1111,7 @@ __global__ void __launch_bounds__(256) k_hou_flatten(const Tube *tubes, const ui
   if (a.keep) a.keep[ib * g.V + i] = root;
 }
 
-__global__ void __launch_bounds__(256) k_hou_apply(const Tube *tubes, const uint32_t *map,
+__global__ void __launch_bounds__(256) k_hou_apply(const Tube *tubes, const uint64_t *map,
                                                    Phase ph) {
   const Where wh = locate(tubes, map);
   const Tube &a = *wh.t;
@@ -1169,9 +1169,9 @@ Plan make_plan(const std::vector<Tube> &tubes) {
   for (size_t ti = 0; ti < tubes.size(); ++ti) {
     const Tube &t = tubes[ti];
     const Lattice &g = t.g;
-    const uint32_t tag = (uint32_t)ti << kMapShift;
+    const uint64_t tag = (uint64_t)ti << 32;
     auto add = [&](int kind, long long count) {
-      for (long long i = 0; i < count; ++i) p.map[kind].push_back(tag | (uint32_t)i);
+      for (long long i = 0; i < count; ++i) p.map[kind].push_back(tag | (uint64_t)i);
     };
     add(kMapSweep, (long long)sweep_blocks_x(g) * g.nt);
     add(kMapEnergy, (long long)energy_blocks_host(g) * g.nt * g.ns);
@@ -1215,11 +1215,11 @@ hipError_t upload_plan(const Plan &p, DevPlan *d) {
   d->lds_slab = p.lds_slab;
   d->identity = p.ntubes == 1;
   if (!total || d->identity) return hipSuccess;
-  hipError_t e = hipMalloc(reinterpret_cast<void **>(&d->buf), total * sizeof(uint32_t));
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&d->buf), total * sizeof(uint64_t));
   if (e != hipSuccess) return e;
   for (int k = 0; k < kMapKinds; ++k)
     if (d->cnt[k]) {
-      e = hipMemcpy(d->buf + d->off[k], p.map[k].data(), d->cnt[k] * sizeof(uint32_t),
+      e = hipMemcpy(d->buf + d->off[k], p.map[k].data(), d->cnt[k] * sizeof(uint64_t),
                     hipMemcpyHostToDevice);
       if (e != hipSuccess) return e;
     }

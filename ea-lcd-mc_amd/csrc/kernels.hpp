@@ -1,8 +1,8 @@
 // kernels.hpp -- launch interface of the gfx950 kernels (internal).
 //
-// Every kernel works on a table of tubes: a block reads a 32-bit entry of a
-// block map (tube number << kMapShift | block index within the tube), then
-// that tube's descriptor.  A single engine is a table of one tube; a batch
+// Every kernel works on a table of tubes: a block reads a 64-bit entry of a
+// block map (tube number << 32 | block index within the tube), then that
+// tube's descriptor.  A single engine is a table of one tube; a batch
 // is a table of many tubes of different sizes advancing in the same launches.
 #pragma once
 
@@ -84,8 +84,6 @@ struct Phase {
   int measure;  // PT / Houdayer: accumulate esum and history
 };
 
-constexpr int kMapShift = 20;  // block map entry: tube << 20 | local block
-
 // Block maps of every kernel kind over one table of tubes.
 // kMapEnergyU / kMapChainU cover only the tubes whose step measures with a
 // separate energy pass (cluster moves on the slab path).
@@ -100,21 +98,21 @@ inline __host__ __device__ bool measured_in_pt(const Tube &t) {
   return !t.cluster || t.planes == 0;
 }
 struct Plan {
-  std::vector<uint32_t> map[kMapKinds];
+  std::vector<uint64_t> map[kMapKinds];
   size_t lds32 = 0, lds64 = 0, lds_slab = 0;
   int ntubes = 0;
 };
 
 // Device copy of a plan: one buffer holding all maps.  The maps of a
 // one-tube table are the identity (entry i = i) and are not stored: map()
-// returns NULL and a block's local index is its block index.
+// returns NULL, the tube is 0 and a block's local index is its block index.
 struct DevPlan {
-  uint32_t *buf = nullptr;
+  uint64_t *buf = nullptr;
   size_t off[kMapKinds] = {};
   size_t cnt[kMapKinds] = {};
   size_t lds32 = 0, lds64 = 0, lds_slab = 0;
   bool identity = false;
-  const uint32_t *map(int k) const { return identity ? nullptr : buf + off[k]; }
+  const uint64_t *map(int k) const { return identity ? nullptr : buf + off[k]; }
   bool ready() const { return identity || buf; }
 };
 

@@ -237,6 +237,35 @@ def test_full_size_houdayer_involution_and_conservation(bc):
         assert np.array_equal(lab[grp, s, b], want), (grp, s, b)
 
 
+def test_full_size_houdayer_31_samples():
+    """L=48 Lz=384 with all 31 samples: 992 cluster instances, so the
+    label-flatten launch has 3456 x 992 > 2^20 workgroups.  Labels of the
+    last instances equal the oracle's, and the move is an involution."""
+    e = full_size_sim(bc=1, ns=31)
+    eng = e.engine
+    e.steps(1, cluster=False)
+    before = e.spins().copy()
+    w = eng.download_perms()
+    pairs = np.array([0, 1, 2, 3], np.int32)
+    eng.keep_labels()
+    eng.houdayer(pairs, 4242, 0)
+    lab = eng.download_labels()
+    say("labels downloaded")
+    L, Lz, ctm = e.L, e.Lz, eng.ctm
+    for (grp, s, b) in [(1, 30, ctm - 1), (0, 20, 3), (1, 15, 0)]:
+        r1, r2 = pairs[2 * grp], pairs[2 * grp + 1]
+        c1, c2 = w[s, r1, b], w[s, r2, b]
+        u1 = eatube.unpack_lattice(L, Lz, e.nr, e.nt, before[s, 0].ravel(),
+                                   before[s, 1].ravel(), r1, c1)
+        u2 = eatube.unpack_lattice(L, Lz, e.nr, e.nt, before[s, 0].ravel(),
+                                   before[s, 1].ravel(), r2, c2)
+        want = orc.labels(L, Lz, 1, u1 ^ u2)
+        assert np.array_equal(lab[grp, s, b], want), (grp, s, b)
+    del lab
+    eng.houdayer(pairs, 4242, 0)
+    assert np.array_equal(e.spins(), before)
+
+
 def test_full_size_pt_matches_oracle_pt():
     """Device PT at 32 temperatures and 124 chains against upd.c's
     restatement fed the same energies and generator states."""
