@@ -543,26 +543,28 @@ struct Target {
 
 // One step of main()'s `it` loop (main.cu:149-228) for every tube of the
 // table: mesfr x { houfr x { ptfr sweeps, PT }, Houdayer }, then the energy
-// measure of tubes that do not measure inside PT, and the step counter.
-// Every PT follows a sweep, so PT takes its energies from the sweep's
-// partials.
-static int enqueue_step(const Target &x, const StepShape &sh) {
+// measure of tubes that do not measure inside PT.  Every PT follows a
+// sweep, so PT takes its energies from the sweep's partials.  With step >= 0
+// (one tube launched step by step) the launches carry the step index;
+// with step < 0 (a captured step, replayed) they read the tubes' step
+// counters, which the step's last launch advances.
+static int enqueue_step(const Target &x, const StepShape &sh, int step = -1) {
   int half = 0;
   for (int im = 0; im < sh.mesfr; ++im) {
     const bool last_m = im == sh.mesfr - 1;
     for (int ih = 0; ih < sh.houfr; ++ih) {
       for (int sw = 0; sw < sh.ptfr; ++sw)
         for (int p = 0; p < 2; ++p) {
-          Phase ph{half++, sw == sh.ptfr - 1 && p == 1, 0, 0};
+          Phase ph{half++, sw == sh.ptfr - 1 && p == 1, 0, 0, step};
           Timed tm(x.timer, EA_K_SWEEP);
           HIP_TRY(launch_sweep(x.dt, *x.plan, ph, x.stream));
         }
-      Phase ph{0, 1, 0, last_m && ih == sh.houfr - 1};
+      Phase ph{0, 1, 0, last_m && ih == sh.houfr - 1, step};
       Timed tm(x.timer, EA_K_PT);
       HIP_TRY(launch_pt(x.dt, *x.plan, ph, x.stream));
     }
     if (sh.cluster) {
-      Phase ph{0, 0, im, last_m};
+      Phase ph{0, 0, im, last_m, step};
       Timed tm(x.timer, EA_K_HOUDAYER);
       HIP_TRY(launch_houdayer(x.dt, *x.plan, ph, x.stream));
     }
@@ -570,12 +572,12 @@ static int enqueue_step(const Target &x, const StepShape &sh) {
   if (sh.unfused) {
     {
       Timed tm(x.timer, EA_K_ENERGY);
-      HIP_TRY(launch_energy(x.dt, *x.plan, true, x.stream));
+      HIP_TRY(launch_energy(x.dt, *x.plan, true, step, x.stream));
     }
     Timed tm(x.timer, EA_K_MEASURE);
-    HIP_TRY(launch_measure(x.dt, *x.plan, true, x.stream));
+    HIP_TRY(launch_measure(x.dt, *x.plan, true, step, x.stream));
   }
-  HIP_TRY(launch_step_end(x.dt, x.ntubes, x.stream));
+  if (step < 0) HIP_TRY(launch_step_end(x.dt, x.ntubes, x.stream));
   return EA_OK;
 }
 
@@ -586,10 +588,10 @@ static Target engine_target(ea_engine *e) { return {e->dt, &e->plan, 1, e->strea
 static int ensure_energy(ea_engine *e) {
   if (e->part_pending) {
     Timed t(e, EA_K_ENERGY);
-    HIP_TRY(launch_resolve(e->dt, e->plan, e->stream));
+    HIP_TRY(launch_resolve(e->dt, e->plan, 0, e->stream));
   } else if (!e->ener_valid) {
     Timed t(e, EA_K_ENERGY);
-    HIP_TRY(launch_energy(e->dt, e->plan, false, e->stream));
+    HIP_TRY(launch_energy(e->dt, e->plan, false, 0, e->stream));
   }
   e->part_pending = false;
   e->ener_valid = true;
@@ -603,7 +605,7 @@ extern "C" int ea_sweep(ea_engine *e, int nsweeps, const ea_counter *blocks) {
   RC_TRY(stage_inputs(e, blocks, (size_t)nsweeps * 2 * e->g().nr, nullptr, 0));
   RC_TRY(begin_calls(e, 1, 2 * nsweeps, 0, 0, 1));
   for (int half = 0; half < 2 * nsweeps; ++half) {
-    Phase ph{half, half == 2 * nsweeps - 1, 0, 0};
+    Phase ph{half, half == 2 * nsweeps - 1, 0, 0, 0};
     Timed t(e, EA_K_SWEEP);
     HIP_TRY(launch_sweep(e->dt, e->plan, ph, e->stream));
   }
@@ -618,7 +620,7 @@ extern "C" int ea_energy(ea_engine *e, int32_t *out) {
   RC_TRY(begin_calls(e, 1, 0, 0, 0, 1));
   {
     Timed t(e, EA_K_ENERGY);
-    HIP_TRY(launch_energy(e->dt, e->plan, false, e->stream));
+    HIP_TRY(launch_energy(e->dt, e->plan, false, 0, e->stream));
   }
   e->part_pending = false;
   e->ener_valid = true;
@@ -633,7 +635,7 @@ extern "C" int ea_energy(ea_engine *e, int32_t *out) {
 extern "C" int ea_pt(ea_engine *e) {
   if (!e) return fail(EA_E_ARG, "engine is NULL");
   RC_TRY(begin_calls(e, 1, 0, 0, 0, 1));
-  Phase ph{0, e->part_pending ? 1 : 0, 0, 0};
+  Phase ph{0, e->part_pending ? 1 : 0, 0, 0, 0};
   if (!e->part_pending) RC_TRY(ensure_energy(e));
   {
     Timed t(e, EA_K_PT);
@@ -682,7 +684,7 @@ extern "C" int ea_houdayer(ea_engine *e, const int32_t pairs[4], uint64_t coin_s
   if (e->t.planes == 0) RC_TRY(ensure_energy(e));  // the whole-tube kernel updates ener
   {
     Timed t(e, EA_K_HOUDAYER);
-    HIP_TRY(launch_houdayer(e->dt, e->plan, Phase{0, 0, 0, 0}, e->stream));
+    HIP_TRY(launch_houdayer(e->dt, e->plan, Phase{0, 0, 0, 0, 0}, e->stream));
   }
   if (e->t.planes > 0) e->part_pending = e->ener_valid = false;
   *rng_offset += e->t.coin_per_call;
@@ -759,7 +761,8 @@ extern "C" int ea_run(ea_engine *e, const ea_schedule *sch, uint64_t *rng_offset
   sh.cluster = sch->cluster != 0;
   sh.unfused = !measured_in_pt(e->t);
   const Target x = engine_target(e);
-  for (int it = 0; it < sch->n_it; ++it) RC_TRY(enqueue_step(x, sh));
+  for (int it = 0; it < sch->n_it; ++it) RC_TRY(enqueue_step(x, sh, it));
+  HIP_TRY(launch_set_step(e->t.step, sch->n_it, e->stream));
   finish_schedule(e, sch, rng_offset_out);
   return EA_OK;
 }

@@ -96,18 +96,18 @@ __device__ __forceinline__ uint32_t qdiv(uint32_t i, uint32_t m) {
 }
 
 // The tube of this block and the block's index within the tube (block map
-// entry tube << 32 | local; no map: tube 0, local = block index); the tube's
-// step counter is read once.
+// entry tube << 32 | local; no map: tube 0, local = block index), and the
+// step: `step` when the launch carries it, else the tube's step counter.
 struct Where {
   const Tube *t;
   int local, step;
 };
-__device__ __forceinline__ Where locate(const Tube *tubes, const uint64_t *map) {
+__device__ __forceinline__ Where locate(const Tube *tubes, const uint64_t *map, int step) {
   const uint64_t e = map ? map[blockIdx.x] : (uint64_t)blockIdx.x;
   Where w;
   w.t = tubes + (e >> 32);
   w.local = (int)(uint32_t)e;
-  w.step = *w.t->step;
+  w.step = step >= 0 ? step : *w.t->step;
   return w;
 }
 
@@ -195,10 +195,10 @@ __device__ __forceinline__ void load_neighbours(const Lattice &g,
   }
 }
 
-// Sum of e over the lanes of this wave holding the same replica
-// (lane & (nr-1)); result valid in every lane.
-__device__ __forceinline__ int replica_sum(int e, int nr) {
-  for (int off = nr; off < 64; off <<= 1) e += __shfl_xor(e, off, 64);
+// Sum of e over the lanes of each `width`-lane group of this wave holding
+// the same replica (lane & (nr-1)); result valid in every lane.
+__device__ __forceinline__ int replica_sum(int e, int nr, int width = 64) {
+  for (int off = nr; off < width; off <<= 1) e += __shfl_xor(e, off, 64);
   return e;
 }
 
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *tubes, const u
   __shared__ uint32_t words[kSwPos][6];
   __shared__ uint64_t ids[kSwPos][2];
   __shared__ uint32_t ctr[EA_MAX_REPLICAS][5];
-  const Where wh = locate(tubes, map);
+  const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *tubes, const u
     if (ph.energy) {
       // every bond has exactly one odd end: the odd half-sweep sees them
       // all; one partial per (sample, replica, beta, block), no atomics
-      e = replica_sum(e, g.nr);
+      e = replica_sum(e, g.nr, kSwPos);
       if (pos < g.nr)
         a.part[(((size_t)s * g.nr + r) * g.nt + beta) * a.nbx + bxi] = e;
     }
@@ -323,8 +323,9 @@ __device__ __forceinline__ int energy_blocks_x(const Lattice &g) {
   return (g.n * g.nr + 255) / 256;
 }
 
-__global__ void __launch_bounds__(256) k_energy(const Tube *tubes, const uint64_t *map) {
-  const Where wh = locate(tubes, map);
+__global__ void __launch_bounds__(256) k_energy(const Tube *tubes, const uint64_t *map,
+                                                int step) {
+  const Where wh = locate(tubes, map, step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
@@ -382,9 +383,10 @@ __device__ __forceinline__ void sum_partials(const int32_t *part, int nt, int nw
 // ener[chain][c] from the chain's partials: the sweep's partials by beta
 // (BY_BETA, clone c = wctb[chain][b]) or k_energy's partials by clone.
 template <bool BY_BETA>
-__global__ void __launch_bounds__(64) k_energy_sum(const Tube *tubes, const uint64_t *map) {
+__global__ void __launch_bounds__(64) k_energy_sum(const Tube *tubes, const uint64_t *map,
+                                                   int step) {
   __shared__ int32_t E[EA_MAX_BETAS];
-  const Where wh = locate(tubes, map);
+  const Where wh = locate(tubes, map, step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const int chain = wh.local, nt = a.g.nt, lane = threadIdx.x;
@@ -427,7 +429,7 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *tubes, const uint64_t *ma
   __shared__ uint8_t perm[EA_MAX_BETAS + 1];
   __shared__ int32_t dE[EA_MAX_BETAS];
   __shared__ uint8_t took[EA_MAX_BETAS];
-  const Where wh = locate(tubes, map);
+  const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
@@ -533,8 +535,9 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *tubes, const uint64_t *ma
 // --------------------------------------------------------------- measures
 
 // Per-beta energy sums and the beta-of-clone history (main.cu:208-228).
-__global__ void __launch_bounds__(64) k_measure(const Tube *tubes, const uint64_t *map) {
-  const Where wh = locate(tubes, map);
+__global__ void __launch_bounds__(64) k_measure(const Tube *tubes, const uint64_t *map,
+                                                int step) {
+  const Where wh = locate(tubes, map, step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
@@ -552,6 +555,10 @@ __global__ void __launch_bounds__(64) k_measure(const Tube *tubes, const uint64_
 __global__ void k_step_end(const Tube *tubes, int ntubes) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < ntubes && *tubes[i].step < tubes[i].nsteps) *tubes[i].step += 1;
+}
+
+__global__ void k_set_step(int *step, int value) {
+  if (threadIdx.x == 0) *step = value;
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
@@ -860,7 +867,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uin
                                                     Phase ph) {
   using R = Rows<RowT>;
   extern __shared__ uint64_t lds[];
-  const Where wh = locate(tubes, map);
+  const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
@@ -1039,7 +1046,7 @@ __device__ __forceinline__ int slab_count(const Tube &a) {
 __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *tubes, const uint64_t *map,
                                                    Phase ph) {
   extern __shared__ uint64_t lds[];
-  const Where wh = locate(tubes, map);
+  const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
@@ -1071,7 +1078,7 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *tubes, const uint
 // the first plane of the slab.
 __global__ void __launch_bounds__(256) k_hou_seam(const Tube *tubes, const uint64_t *map,
                                                   Phase ph) {
-  const Where wh = locate(tubes, map);
+  const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
@@ -1096,7 +1103,7 @@ __device__ __forceinline__ int word_blocks(const Lattice &g) { return (2 * g.n +
 
 __global__ void __launch_bounds__(256) k_hou_flatten(const Tube *tubes, const uint64_t *map,
                                                      Phase ph) {
-  const Where wh = locate(tubes, map);
+  const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
@@ -1113,7 +1120,7 @@ __global__ void __launch_bounds__(256) k_hou_flatten(const Tube *tubes, const ui
 
 __global__ void __launch_bounds__(256) k_hou_apply(const Tube *tubes, const uint64_t *map,
                                                    Phase ph) {
-  const Where wh = locate(tubes, map);
+  const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
@@ -1238,20 +1245,22 @@ hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_energy(const Tube *tubes, const DevPlan &p, bool unfused_only, hipStream_t s) {
+hipError_t launch_energy(const Tube *tubes, const DevPlan &p, bool unfused_only, int step,
+                         hipStream_t s) {
   const int me = unfused_only ? kMapEnergyU : kMapEnergy, mc = unfused_only ? kMapChainU : kMapChain;
   if (p.cnt[me])
-    hipLaunchKernelGGL(k_energy, dim3((unsigned)p.cnt[me]), dim3(256), 0, s, tubes, p.map(me));
+    hipLaunchKernelGGL(k_energy, dim3((unsigned)p.cnt[me]), dim3(256), 0, s, tubes, p.map(me),
+                       step);
   if (p.cnt[mc])
     hipLaunchKernelGGL(k_energy_sum<false>, dim3((unsigned)p.cnt[mc]), dim3(64), 0, s, tubes,
-                       p.map(mc));
+                       p.map(mc), step);
   return hipGetLastError();
 }
 
-hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, hipStream_t s) {
+hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, int step, hipStream_t s) {
   if (p.cnt[kMapChain])
     hipLaunchKernelGGL(k_energy_sum<true>, dim3((unsigned)p.cnt[kMapChain]), dim3(64), 0, s,
-                       tubes, p.map(kMapChain));
+                       tubes, p.map(kMapChain), step);
   return hipGetLastError();
 }
 
@@ -1262,10 +1271,17 @@ hipError_t launch_pt(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_measure(const Tube *tubes, const DevPlan &p, bool unfused_only, hipStream_t s) {
+hipError_t launch_measure(const Tube *tubes, const DevPlan &p, bool unfused_only, int step,
+                          hipStream_t s) {
   const int mc = unfused_only ? kMapChainU : kMapChain;
   if (p.cnt[mc])
-    hipLaunchKernelGGL(k_measure, dim3((unsigned)p.cnt[mc]), dim3(64), 0, s, tubes, p.map(mc));
+    hipLaunchKernelGGL(k_measure, dim3((unsigned)p.cnt[mc]), dim3(64), 0, s, tubes, p.map(mc),
+                       step);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_step(int *step, int value, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, step, value);
   return hipGetLastError();
 }
 

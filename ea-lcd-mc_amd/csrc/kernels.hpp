@@ -78,10 +78,12 @@ struct Tube {
 
 // Position of a launch inside a step, fixed when the step is enqueued.
 struct Phase {
-  int half;     // half-sweep index within the step (sweep), parity = half & 1
-  int energy;   // sweep: write the energy partials
-  int call;     // Houdayer call index within the step
-  int measure;  // PT / Houdayer: accumulate esum and history
+  int half;        // half-sweep index within the step (sweep), parity = half & 1
+  int energy;      // sweep: write the energy partials
+  int call;        // Houdayer call index within the step
+  int measure;     // PT / Houdayer: accumulate esum and history
+  int step = -1;   // the step, when the host knows it (one tube, launched
+                   // step by step); -1: read the tube's step counter
 };
 
 // Block maps of every kernel kind over one table of tubes.
@@ -128,12 +130,15 @@ void free_plan(DevPlan *d);
 
 // Launches over a device tube table.  Each returns hipGetLastError().
 hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
-hipError_t launch_energy(const Tube *tubes, const DevPlan &p, bool unfused_only, hipStream_t s);
-hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, hipStream_t s);
+hipError_t launch_energy(const Tube *tubes, const DevPlan &p, bool unfused_only, int step,
+                         hipStream_t s);
+hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_pt(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
 hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
-hipError_t launch_measure(const Tube *tubes, const DevPlan &p, bool unfused_only, hipStream_t s);
+hipError_t launch_measure(const Tube *tubes, const DevPlan &p, bool unfused_only, int step,
+                          hipStream_t s);
 hipError_t launch_step_end(const Tube *tubes, int ntubes, hipStream_t s);
+hipError_t launch_set_step(int *step, int value, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
 
