@@ -166,8 +166,11 @@ def batched_compare(steps, samples=1, ntubes=48):
     for p in plain:
         p.engine.sync()
     t_plain = time.perf_counter() - t0
-    # batched form
+    # batched form, in the launch shape the autotuner finds fastest for
+    # this mix (timed on scratch tubes of the same configurations)
+    shape, shape_times = eatube.tune_shape([t.engine for t in twins], steps=10)
     batch = eatube.Batch([t.engine for t in twins])
+    batch.set_shape(*shape)
     batch.run([t.schedule(1, cluster=True) for t in twins])
     scheds = [t.schedule(steps, cluster=True) for t in twins]
     t0 = time.perf_counter()
@@ -185,6 +188,9 @@ def batched_compare(steps, samples=1, ntubes=48):
             "plain_value": attempts / (t_plain * 1e9),
             "batched_value": attempts / (t_batch * 1e9),
             "speedup": t_plain / t_batch, "bit_exact": bool(same),
+            "launch_shape": list(shape),
+            "tuner_ms_per_step": {"%d/%d" % k: 1e3 * v / 10
+                                  for k, v in shape_times.items()},
             "unit": "spin-flip attempts/ns"}
 
 
@@ -242,6 +248,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shape (64, 1024)")
     ap.add_argument("--timing-steps", type=int, default=50)
     ap.add_argument("--batch-steps", type=int, default=200,
                     help="steps of the 48-tube mixed batch comparison (0: skip)")
@@ -281,6 +289,13 @@ def main():
                             betas=betas, seeds=SEEDS)
     eng = sim.engine
     ctm = eng.ctm
+    shape, shape_times = (64, 1024), {}
+    if not args.no_tune:
+        shape, shape_times = eatube.tune_shape([eng], cluster=w["cluster"],
+                                               steps=50)
+        log("launch shape", shape, {k: round(v * 20, 4)
+                                    for k, v in shape_times.items()})
+    eng.set_shape(*shape)
     log("warmup", args.warmup)
     sim.steps(args.warmup, cluster=w["cluster"])
     eng.sync()
@@ -336,6 +351,10 @@ def main():
                    "step": "sweep+energy+PT" + ("+Houdayer" if w["cluster"]
                                                 else "") + "+measure"},
         "parity_vs_oracle": parity,
+        "launch_shape": {"sweep_positions": shape[0],
+                         "cluster_threads": shape[1],
+                         "tuner_ms_per_step": {"%d/%d" % k: 1e3 * v / 50
+                                               for k, v in shape_times.items()}},
         "kernels": kern,
         "roofline": roof,
     }

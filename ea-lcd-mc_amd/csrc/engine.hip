@@ -77,6 +77,7 @@ struct ea_engine {
   Tube t{};             // host copy of the descriptor
   Tube *dt = nullptr;   // device copy
   DevPlan plan;
+  Shape shape;          // launch shape of the engine's own calls
   uint32_t job = 0;
   std::vector<double> betas;
   size_t ctr_cap = 0, pairs_cap = 0;
@@ -134,6 +135,7 @@ struct ea_batch {
   Tube *dt = nullptr;
   DevPlan plan;
   std::vector<int> plan_cluster;  // cluster flags the plan was built for
+  Shape launch;                   // launch shape asked for by the caller
   StepShape shape;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -292,7 +294,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   g.ctm = cfg->clone_threshold;
   g.half = (int64_t)g.n * g.nr * g.nt;
   set_divisors(g);
-  t.nbx = sweep_blocks_x(g);
+  t.nbx = sweep_blocks_x(g, e->shape.sweep_positions);
   t.frec = 1;
   t.cluster = g.ctm > 0 && g.nr >= 4;
   e->job = cfg->job_id;
@@ -369,7 +371,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
         return rc;
     }
   }
-  HIP_TRY(upload_plan(make_plan({t}), &e->plan));
+  HIP_TRY(upload_plan(make_plan({t}, e->shape), &e->plan));
   RC_TRY(upload_tube(e.get()));
   *out = e.release();
   return EA_OK;
@@ -665,7 +667,31 @@ static int set_cluster(ea_engine *e, int cluster) {
   if (e->t.cluster == cluster) return EA_OK;
   e->t.cluster = cluster;
   HIP_TRY(hipStreamSynchronize(e->stream));
-  HIP_TRY(upload_plan(make_plan({e->t}), &e->plan));
+  HIP_TRY(upload_plan(make_plan({e->t}, e->shape), &e->plan));
+  return EA_OK;
+}
+
+static int check_shape(const ea_shape *sh) {
+  if (!sh) return fail(EA_E_ARG, "NULL argument");
+  if (sh->sweep_positions != 32 && sh->sweep_positions != 64)
+    return fail(EA_E_ARG, "sweep_positions must be 32 or 64 (got %d)", sh->sweep_positions);
+  if (sh->cluster_threads != 256 && sh->cluster_threads != 512 && sh->cluster_threads != 1024)
+    return fail(EA_E_ARG, "cluster_threads must be 256, 512 or 1024 (got %d)",
+                sh->cluster_threads);
+  return EA_OK;
+}
+
+// Partials by sweep block change layout with the shape: pending ones are
+// summed first, under the old shape.
+extern "C" int ea_set_shape(ea_engine *e, const ea_shape *shape) {
+  if (!e) return fail(EA_E_ARG, "NULL engine");
+  RC_TRY(check_shape(shape));
+  if (e->part_pending) RC_TRY(ensure_energy(e));
+  e->shape.sweep_positions = shape->sweep_positions;
+  e->shape.cluster_threads = shape->cluster_threads;
+  e->t.nbx = sweep_blocks_x(e->g(), e->shape.sweep_positions);
+  RC_TRY(upload_tube(e));
+  HIP_TRY(upload_plan(make_plan({e->t}, e->shape), &e->plan));
   return EA_OK;
 }
 
@@ -845,6 +871,15 @@ static int drop_graph(ea_batch *b) {
   return EA_OK;
 }
 
+// Takes effect at the next ea_batch_run (the plan and graph are rebuilt).
+extern "C" int ea_batch_set_shape(ea_batch *b, const ea_shape *shape) {
+  if (!b) return fail(EA_E_ARG, "NULL batch");
+  RC_TRY(check_shape(shape));
+  b->launch.sweep_positions = shape->sweep_positions;
+  b->launch.cluster_threads = shape->cluster_threads;
+  return EA_OK;
+}
+
 extern "C" int ea_batch_run(ea_batch *b, const ea_schedule *sch, uint64_t *rng_offsets_out) {
   if (!b || !sch) return fail(EA_E_ARG, "NULL argument");
   const int n = (int)b->eng.size();
@@ -866,15 +901,16 @@ extern "C" int ea_batch_run(ea_batch *b, const ea_schedule *sch, uint64_t *rng_o
     ea_engine *e = b->eng[i];
     RC_TRY(load_schedule(e, &sch[i]));
     tubes[i] = e->t;
+    tubes[i].nbx = sweep_blocks_x(e->g(), b->launch.sweep_positions);
     cluster[i] = e->t.cluster;
     sh.cluster = sh.cluster || e->t.cluster;
     sh.unfused = sh.unfused || !measured_in_pt(e->t);
   }
   HIP_TRY(hipStreamSynchronize(b->stream));
   HIP_TRY(hipMemcpy(b->dt, tubes.data(), sizeof(Tube) * n, hipMemcpyHostToDevice));
-  if (cluster != b->plan_cluster || !b->plan.ready()) {
+  if (cluster != b->plan_cluster || !b->plan.ready() || !(b->plan.shape == b->launch)) {
     RC_TRY(drop_graph(b));
-    HIP_TRY(upload_plan(make_plan(tubes), &b->plan));
+    HIP_TRY(upload_plan(make_plan(tubes, b->launch), &b->plan));
     b->plan_cluster = cluster;
   }
   if (!b->exec || !(sh == b->shape)) {

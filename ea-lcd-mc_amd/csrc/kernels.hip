@@ -204,19 +204,21 @@ __device__ __forceinline__ int replica_sum(int e, int nr, int width = 64) {
 
 // ------------------------------------------------------------------ sweep
 
-// One half-sweep of every sample.  A block owns 64 consecutive (h, r)
-// positions of one beta.  First 384 of its threads each make one of the six
-// random words of the reference thread tid = beta*n*nr + h*nr + r
-// (d_gen_rndbits_cuda, main.cu:1095-1150) with the beta's LUT in LDS; then
-// its 16 waves update 16 samples at a time, each wave the clone that sits at
-// beta in its sample (d_oneMCstepBN_multisample, main.cu:1385-1510).  The
-// random words are shared by all samples, are made once and never reach HBM.
-constexpr int kSwPos = 64;                  // positions per block (one wave)
-constexpr int kSwSlots = 16;                // samples in flight per position
-constexpr int kSwThreads = kSwPos * kSwSlots;
+// One half-sweep of every sample.  A block of 1024 threads owns POS (64 or
+// 32) consecutive (h, r) positions of one beta.  First 6*POS of its threads
+// each make one of the six random words of the reference thread
+// tid = beta*n*nr + h*nr + r (d_gen_rndbits_cuda, main.cu:1095-1150) with the
+// beta's LUT in LDS; then each group of POS lanes updates one sample,
+// 1024/POS samples at a time, each the clone that sits at beta in its sample
+// (d_oneMCstepBN_multisample, main.cu:1385-1510).  The random words are
+// shared by all samples, are made once and never reach HBM.
+constexpr int kSwThreads = 1024;
 
+template <int POS>
 __global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *tubes, const uint64_t *map,
                                                       Phase ph) {
+  static_assert(POS == 32 || POS == 64, "a sample's positions stay within one wave");
+  constexpr int kSwPos = POS, kSwSlots = kSwThreads / POS;
   __shared__ uint32_t lut[EA_LUT_BYTES / 4];
   __shared__ uint8_t clone_at[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
   __shared__ uint32_t words[kSwPos][6];
@@ -1152,13 +1154,16 @@ inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) /
 
 // ------------------------------------------------------------- launchers
 
-int sweep_blocks_x(const Lattice &g) { return (int)blocks_for((long long)g.n * g.nr, kSwPos); }
+int sweep_blocks_x(const Lattice &g, int positions) {
+  return (int)blocks_for((long long)g.n * g.nr, positions);
+}
 
 static int energy_blocks_host(const Lattice &g) { return (int)blocks_for((long long)g.n * g.nr, 256); }
 
 size_t energy_partials(const Lattice &g) {
   const size_t per = (size_t)g.ns * g.nr * g.nt;
-  return per * std::max<size_t>(sweep_blocks_x(g), 4 * (size_t)energy_blocks_host(g));
+  return per * std::max<size_t>(sweep_blocks_x(g, kSweepPosMin),
+                                4 * (size_t)energy_blocks_host(g));
 }
 
 size_t hou_whole_lds(const Lattice &g) {
@@ -1170,9 +1175,10 @@ size_t hou_slab_lds(const Lattice &g, int planes) {
   return 8 * (size_t)planes * g.L + 4 * (size_t)planes * g.L * g.L;
 }
 
-Plan make_plan(const std::vector<Tube> &tubes) {
+Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
   Plan p;
   p.ntubes = (int)tubes.size();
+  p.shape = shape;
   for (size_t ti = 0; ti < tubes.size(); ++ti) {
     const Tube &t = tubes[ti];
     const Lattice &g = t.g;
@@ -1180,7 +1186,7 @@ Plan make_plan(const std::vector<Tube> &tubes) {
     auto add = [&](int kind, long long count) {
       for (long long i = 0; i < count; ++i) p.map[kind].push_back(tag | (uint64_t)i);
     };
-    add(kMapSweep, (long long)sweep_blocks_x(g) * g.nt);
+    add(kMapSweep, (long long)t.nbx * g.nt);
     add(kMapEnergy, (long long)energy_blocks_host(g) * g.nt * g.ns);
     add(kMapChain, (long long)g.ns * g.nr);
     if (!t.cluster || g.ctm <= 0) continue;
@@ -1220,6 +1226,7 @@ hipError_t upload_plan(const Plan &p, DevPlan *d) {
   d->lds32 = p.lds32;
   d->lds64 = p.lds64;
   d->lds_slab = p.lds_slab;
+  d->shape = p.shape;
   d->identity = p.ntubes == 1;
   if (!total || d->identity) return hipSuccess;
   hipError_t e = hipMalloc(reinterpret_cast<void **>(&d->buf), total * sizeof(uint64_t));
@@ -1239,9 +1246,12 @@ void free_plan(DevPlan *d) {
 }
 
 hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s) {
-  if (p.cnt[kMapSweep])
-    hipLaunchKernelGGL(k_sweep, dim3((unsigned)p.cnt[kMapSweep]), dim3(kSwThreads), 0, s,
-                       tubes, p.map(kMapSweep), ph);
+  if (!p.cnt[kMapSweep]) return hipGetLastError();
+  const dim3 grid((unsigned)p.cnt[kMapSweep]);
+  if (p.shape.sweep_positions == 32)
+    hipLaunchKernelGGL(k_sweep<32>, grid, dim3(kSwThreads), 0, s, tubes, p.map(kMapSweep), ph);
+  else
+    hipLaunchKernelGGL(k_sweep<64>, grid, dim3(kSwThreads), 0, s, tubes, p.map(kMapSweep), ph);
   return hipGetLastError();
 }
 
@@ -1312,13 +1322,13 @@ static void allow_lds(const void *kernel, size_t bytes) {
 hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s) {
   if (p.cnt[kMapWhole32]) {
     allow_lds(reinterpret_cast<const void *>(k_hou_whole<uint32_t>), p.lds32);
-    hipLaunchKernelGGL(k_hou_whole<uint32_t>, dim3((unsigned)p.cnt[kMapWhole32]), dim3(1024),
-                       p.lds32, s, tubes, p.map(kMapWhole32), ph);
+    hipLaunchKernelGGL(k_hou_whole<uint32_t>, dim3((unsigned)p.cnt[kMapWhole32]),
+                       dim3(p.shape.cluster_threads), p.lds32, s, tubes, p.map(kMapWhole32), ph);
   }
   if (p.cnt[kMapWhole64]) {
     allow_lds(reinterpret_cast<const void *>(k_hou_whole<uint64_t>), p.lds64);
-    hipLaunchKernelGGL(k_hou_whole<uint64_t>, dim3((unsigned)p.cnt[kMapWhole64]), dim3(1024),
-                       p.lds64, s, tubes, p.map(kMapWhole64), ph);
+    hipLaunchKernelGGL(k_hou_whole<uint64_t>, dim3((unsigned)p.cnt[kMapWhole64]),
+                       dim3(p.shape.cluster_threads), p.lds64, s, tubes, p.map(kMapWhole64), ph);
   }
   if (p.cnt[kMapSlab]) {
     allow_lds(reinterpret_cast<const void *>(k_hou_slab), p.lds_slab);

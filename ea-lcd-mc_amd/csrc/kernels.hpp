@@ -99,8 +99,21 @@ enum MapKind {
 inline __host__ __device__ bool measured_in_pt(const Tube &t) {
   return !t.cluster || t.planes == 0;
 }
+// Launch shape of a table's kernels (chosen by measurement, ea_set_shape /
+// ea_batch_set_shape): sweep positions per 1024-thread block (32 or 64) and
+// threads of a whole-tube Houdayer workgroup (256, 512 or 1024).
+struct Shape {
+  int sweep_positions = 64;
+  int cluster_threads = 1024;
+  bool operator==(const Shape &o) const {
+    return sweep_positions == o.sweep_positions && cluster_threads == o.cluster_threads;
+  }
+};
+constexpr int kSweepPosMin = 32;
+
 struct Plan {
   std::vector<uint64_t> map[kMapKinds];
+  Shape shape;
   size_t lds32 = 0, lds64 = 0, lds_slab = 0;
   int ntubes = 0;
 };
@@ -113,18 +126,21 @@ struct DevPlan {
   size_t off[kMapKinds] = {};
   size_t cnt[kMapKinds] = {};
   size_t lds32 = 0, lds64 = 0, lds_slab = 0;
+  Shape shape;
   bool identity = false;
   const uint64_t *map(int k) const { return identity ? nullptr : buf + off[k]; }
   bool ready() const { return identity || buf; }
 };
 
-// Sweep blocks per beta, and partial-energy slots a tube needs.
-int sweep_blocks_x(const Lattice &g);
+// Sweep blocks per beta for a block of `positions` positions, and the
+// partial-energy slots a tube needs under any shape.
+int sweep_blocks_x(const Lattice &g, int positions);
 size_t energy_partials(const Lattice &g);
 
 // Block maps of the tubes (host copies of the descriptors give geometry,
 // Houdayer path and whether the tube has cluster moves).
-Plan make_plan(const std::vector<Tube> &tubes);
+// The tubes' nbx must be sweep_blocks_x(g, shape.sweep_positions).
+Plan make_plan(const std::vector<Tube> &tubes, Shape shape);
 hipError_t upload_plan(const Plan &p, DevPlan *d);
 void free_plan(DevPlan *d);
 

@@ -6,8 +6,10 @@ The engine is HIP code for gfx950; nothing here computes physics in Python.
 Loading fails loudly when the library has not been built.
 """
 import ctypes as C
+import json
 import math
 import os
+import time
 
 import numpy as np
 
@@ -53,6 +55,10 @@ class Schedule(C.Structure):
                 ("coin_seed", C.c_uint64), ("rng_offset", C.c_uint64)]
 
 
+class Shape(C.Structure):
+    _fields_ = [("sweep_positions", C.c_int32), ("cluster_threads", C.c_int32)]
+
+
 assert C.sizeof(HQ) == 2080 and C.sizeof(XO) == 48
 
 EXPORTS = [
@@ -66,7 +72,7 @@ EXPORTS = [
     "ea_houdayer_pairs", "ea_unpack_lattice", "ea_set_timing",
     "ea_read_timing", "ea_batch_create", "ea_batch_destroy", "ea_batch_run",
     "ea_footprint", "ea_select_device", "ea_pack_lattice", "ea_conf_bytes",
-    "ea_conf_encode", "ea_conf_decode",
+    "ea_conf_encode", "ea_conf_decode", "ea_set_shape", "ea_batch_set_shape",
 ]
 KINDS = ("sweep", "energy", "pt", "houdayer", "measure")
 
@@ -117,6 +123,8 @@ def lib():
         "ea_batch_destroy": (C.c_int, [vp]),
         "ea_batch_run": (C.c_int, [vp, p(Schedule), p(u64)]),
         "ea_footprint": (C.c_int, [p(Config), p(u64)]),
+        "ea_set_shape": (C.c_int, [vp, p(Shape)]),
+        "ea_batch_set_shape": (C.c_int, [vp, p(Shape)]),
         "ea_select_device": (C.c_int, [C.c_int]),
         "ea_pack_lattice": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int,
                                       p(C.c_uint8), C.c_int, C.c_int, p(u64),
@@ -343,6 +351,11 @@ class Engine:
         check(lib().ea_run(self.h, C.byref(sched), C.byref(off)))
         return off.value
 
+    def set_shape(self, sweep_positions=64, cluster_threads=1024):
+        """Launch shape of this engine's calls (results do not change)."""
+        sh = Shape(sweep_positions, cluster_threads)
+        check(lib().ea_set_shape(self.h, C.byref(sh)))
+
     def set_timing(self, on=True):
         check(lib().ea_set_timing(self.h, int(on)))
 
@@ -378,6 +391,11 @@ class Batch:
         off = np.zeros(len(schedules), np.uint64)
         check(lib().ea_batch_run(self.h, arr, _ptr(off, C.c_uint64)))
         return off
+
+    def set_shape(self, sweep_positions=64, cluster_threads=1024):
+        """Launch shape of the batch's steps, from the next run on."""
+        sh = Shape(sweep_positions, cluster_threads)
+        check(lib().ea_batch_set_shape(self.h, C.byref(sh)))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -487,3 +505,76 @@ class Simulation:
         for i in range(self.ns):
             out[i, 0], out[i, 1] = e.download_spins(i)
         return out.reshape(self.ns, 2, self.nt, e.n, self.nr)
+
+
+# -------------------------------------------------------------- autotuner
+
+# Launch shapes the engine takes: (sweep positions per block, threads of a
+# whole-tube Houdayer workgroup).
+SHAPES = ((64, 1024), (32, 1024), (64, 512), (32, 512), (64, 256), (32, 256))
+
+
+def tube_key(engine):
+    e = engine
+    return [e.L, e.Lz, e.nr, e.nt, e.ns, e.bc, e.ctm]
+
+
+def shape_signature(engines, ptfr=1, houfr=1, mesfr=1, cluster=True):
+    """Cache key of a tube mix: the sorted tube configurations and the step
+    frequencies (the launch sequence depends on nothing else)."""
+    tubes = sorted(tube_key(e) for e in engines)
+    return json.dumps({"tubes": tubes, "freq": [ptfr, houfr, mesfr],
+                       "cluster": int(bool(cluster))}, separators=(",", ":"))
+
+
+def load_shape_cache(path):
+    if path and os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def save_shape_cache(path, cache):
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def tune_shape(engines, ptfr=1, houfr=1, mesfr=1, cluster=True, steps=8,
+               shapes=SHAPES, cache=None):
+    """Fastest launch shape for advancing `engines` together, by timing.
+
+    The candidates run on scratch twins (engines of the same configurations,
+    betas and LUTs, with samples of their own), so the engines themselves
+    are not touched: each shape gets one untimed step (plan and graph are
+    rebuilt) and then `steps` timed steps of one batch of the twins.
+    Returns ((sweep_positions, cluster_threads), {shape: seconds}); with a
+    `cache` dict holding the mix's signature, returns its entry without
+    timing anything, and otherwise records the winner there."""
+    key = shape_signature(engines, ptfr, houfr, mesfr, cluster)
+    if cache is not None and key in cache:
+        return tuple(cache[key]), {}
+    twins = [Simulation(e.L, e.Lz, e.nr, e.nt, e.ns, bc=e.bc, betas=e.betas,
+                        lut=e.lut, clone_thr=e.ctm,
+                        seeds=(101 + i, 202 + i, 303 + i, 404 + i))
+             for i, e in enumerate(engines)]
+    batch = Batch([t.engine for t in twins])
+    kw = dict(ptfr=ptfr, houfr=houfr, mesfr=mesfr, cluster=cluster)
+    times = {}
+    try:
+        for shp in shapes:
+            batch.set_shape(*shp)
+            batch.run([t.schedule(1, **kw) for t in twins])
+            sch = [t.schedule(steps, **kw) for t in twins]
+            t0 = time.perf_counter()
+            batch.run(sch)
+            times[tuple(shp)] = time.perf_counter() - t0
+    finally:
+        batch.close()
+        for t in twins:
+            t.engine.close()
+    best = min(times, key=times.get)
+    if cache is not None:
+        cache[key] = list(best)
+    return best, times

@@ -233,6 +233,20 @@ int ea_batch_destroy(ea_batch *b);
  * receives engine i's coin offset after its last Houdayer call. */
 int ea_batch_run(ea_batch *b, const ea_schedule *schedules, uint64_t *rng_offsets_out);
 
+/* Launch shape (no counterpart in the reference, whose block sizes are
+ * compile-time constants, main.cu:1095-1510): positions per 1024-thread
+ * sweep block (32 or 64) and threads of a whole-tube Houdayer workgroup
+ * (256, 512 or 1024).  Results do not depend on it, only speed does; the
+ * best shape of a tube mix is found by timing (eatube.tune_shape).  The
+ * default is {64, 1024}.  ea_set_shape applies to an engine's own calls,
+ * ea_batch_set_shape to the batch's launches. */
+typedef struct ea_shape {
+  int32_t sweep_positions;
+  int32_t cluster_threads;
+} ea_shape;
+int ea_set_shape(ea_engine *e, const ea_shape *shape);
+int ea_batch_set_shape(ea_batch *b, const ea_shape *shape);
+
 /* ------------------------------------------------- host helpers (no GPU) */
 
 /* Init_Rand_HQ_64bits (random_generator.cu:160-201) and one draw

@@ -108,3 +108,55 @@ def test_batch_matches_oracle():
         es, _ = s.engine.read_measures()
         assert np.array_equal(es.ravel(), o.field("esum")), "tube %d esum" % k
     batch.close()
+
+
+def test_every_launch_shape_gives_the_same_state():
+    """Launch shapes change speed only: each shape of the batch and of a
+    single engine ends bit-identical to the default shape."""
+    specs = [(8, 16, 0, 6, 2), (16, 64, 1, 4, 1), (24, 24, 0, 4, 1)]
+    ref = [make(s, 30 + k) for k, s in enumerate(specs)]
+    for p in ref:
+        p.steps(2, cluster=True)
+    want = [state(p) for p in ref]
+    for shp in eatube.SHAPES:
+        say("batch shape", shp)
+        twins = [make(s, 30 + k) for k, s in enumerate(specs)]
+        batch = eatube.Batch([t.engine for t in twins])
+        batch.set_shape(*shp)
+        batch.run([t.schedule(2, cluster=True) for t in twins])
+        for k, t in enumerate(twins):
+            same(want[k], state(t), "batch shape %s tube %d" % (shp, k))
+        batch.close()
+    for shp in eatube.SHAPES:
+        say("engine shape", shp)
+        solo = make(specs[1], 31)
+        solo.engine.set_shape(*shp)
+        solo.steps(1, cluster=True)
+        solo.engine.sweep(solo.counters(2))      # leaves partials pending
+        other = (32 if shp[0] == 64 else 64, shp[1])
+        solo.engine.set_shape(*other)            # across a pending sweep
+        solo.engine.pt()
+        p = make(specs[1], 31)
+        p.steps(1, cluster=True)
+        p.engine.sweep(p.counters(2))
+        p.engine.pt()
+        same(state(p), state(solo), "engine shape %s" % (shp,))
+
+
+def test_tuner_picks_a_shape_and_caches_it(tmp_path):
+    specs = [(8, 16, 0, 6, 2), (16, 32, 1, 4, 1)]
+    sims = [make(s, 40 + k) for k, s in enumerate(specs)]
+    before = [state(s) for s in sims]
+    cache = {}
+    best, times = eatube.tune_shape([s.engine for s in sims], steps=2,
+                                    cache=cache)
+    say("tuned", best, times)
+    assert best in eatube.SHAPES and set(times) == set(eatube.SHAPES)
+    assert min(times.values()) == times[best]
+    for b, s in zip(before, sims):     # the engines themselves are untouched
+        same(b, state(s), "after tuning")
+    path = str(tmp_path / "shapes.json")
+    eatube.save_shape_cache(path, cache)
+    again, t2 = eatube.tune_shape([s.engine for s in sims[::-1]],
+                                  cache=eatube.load_shape_cache(path))
+    assert again == best and t2 == {}

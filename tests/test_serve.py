@@ -112,3 +112,25 @@ def test_poisson_requests_shapes():
     assert abs(mean_gap - 1 / 50.0) < 0.3 / 50.0
     lg = [math.log(r.sweeps) for r in rs]
     assert abs(sum(lg) / len(lg) - (math.log(1000) + math.log(100000)) / 2) < 0.4
+
+
+def test_shape_signature_and_cache(tmp_path):
+    import eatube
+
+    class E:
+        def __init__(self, *k):
+            self.L, self.Lz, self.nr, self.nt, self.ns, self.bc, self.ctm = k
+
+    a, b = E(8, 16, 4, 6, 2, 0, 3), E(16, 64, 4, 4, 1, 1, 2)
+    assert eatube.shape_signature([a, b]) == eatube.shape_signature([b, a])
+    assert eatube.shape_signature([a, b]) != eatube.shape_signature([a, b],
+                                                                    ptfr=2)
+    assert eatube.shape_signature([a]) != eatube.shape_signature([a],
+                                                                 cluster=False)
+    path = str(tmp_path / "s.json")
+    assert eatube.load_shape_cache(path) == {}
+    eatube.save_shape_cache(path, {"k": [32, 512]})
+    assert eatube.load_shape_cache(path) == {"k": [32, 512]}
+    # a cached mix is answered without building anything
+    cache = {eatube.shape_signature([a, b]): [32, 512]}
+    assert eatube.tune_shape([b, a], cache=cache) == ((32, 512), {})
