@@ -189,7 +189,7 @@ def batched_compare(steps, samples=1, ntubes=48):
             "batched_value": attempts / (t_batch * 1e9),
             "speedup": t_plain / t_batch, "bit_exact": bool(same),
             "launch_shape": list(shape),
-            "tuner_ms_per_step": {"%d/%d" % k: 1e3 * v / 10
+            "tuner_ms_per_step": {"/".join(map(str, k)): 1e3 * v / 10
                                   for k, v in shape_times.items()},
             "unit": "spin-flip attempts/ns"}
 
@@ -249,7 +249,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
-                    help="keep the default launch shape (64, 1024)")
+                    help="keep the default launch shape")
     ap.add_argument("--timing-steps", type=int, default=50)
     ap.add_argument("--batch-steps", type=int, default=200,
                     help="steps of the 48-tube mixed batch comparison (0: skip)")
@@ -289,7 +289,7 @@ def main():
                             betas=betas, seeds=SEEDS)
     eng = sim.engine
     ctm = eng.ctm
-    shape, shape_times = (64, 1024), {}
+    shape, shape_times = eatube.DEFAULT_SHAPE, {}
     if not args.no_tune:
         shape, shape_times = eatube.tune_shape([eng], cluster=w["cluster"],
                                                steps=50)
@@ -353,7 +353,8 @@ def main():
         "parity_vs_oracle": parity,
         "launch_shape": {"sweep_positions": shape[0],
                          "cluster_threads": shape[1],
-                         "tuner_ms_per_step": {"%d/%d" % k: 1e3 * v / 50
+                         "sweep_kind": shape[2], "sweep_xcd": shape[3],
+                         "tuner_ms_per_step": {"/".join(map(str, k)): 1e3 * v / 50
                                                for k, v in shape_times.items()}},
         "kernels": kern,
         "roofline": roof,

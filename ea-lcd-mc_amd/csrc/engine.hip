@@ -294,7 +294,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   g.ctm = cfg->clone_threshold;
   g.half = (int64_t)g.n * g.nr * g.nt;
   set_divisors(g);
-  t.nbx = sweep_blocks_x(g, e->shape.sweep_positions);
+  t.nbx = sweep_partials_x(g, e->shape);
   t.frec = 1;
   t.cluster = g.ctm > 0 && g.nr >= 4;
   e->job = cfg->job_id;
@@ -430,21 +430,38 @@ extern "C" int ea_set_keys(ea_engine *e, const uint32_t *keys) {
   return upload_tube(e);
 }
 
+// One parity array of one sample: the reference's [clone][h][replica] words
+// to the device's [replica][clone][h] rows and back.
+static void spins_to_device(const Lattice &g, const uint64_t *host, uint64_t *dev) {
+  for (int c = 0; c < g.nt; ++c)
+    for (int h = 0; h < g.n; ++h)
+      for (int r = 0; r < g.nr; ++r)
+        dev[spin_row(g, r, c) + h] = host[((size_t)c * g.n + h) * g.nr + r];
+}
+static void spins_from_device(const Lattice &g, const uint64_t *dev, uint64_t *host) {
+  for (int c = 0; c < g.nt; ++c)
+    for (int h = 0; h < g.n; ++h)
+      for (int r = 0; r < g.nr; ++r)
+        host[((size_t)c * g.n + h) * g.nr + r] = dev[spin_row(g, r, c) + h];
+}
+
 extern "C" int ea_upload_sample(ea_engine *e, int s, const uint64_t *even,
                                 const uint64_t *odd, const uint64_t *J) {
   if (!e || !even || !odd || !J) return fail(EA_E_ARG, "NULL argument");
   const Lattice &g = e->g();
   if (s < 0 || s >= g.ns) return fail(EA_E_ARG, "sample %d out of range", s);
   HIP_TRY(hipStreamSynchronize(e->stream));
-  uint64_t *base = e->t.spin + (size_t)s * 2 * g.half;
-  HIP_TRY(hipMemcpy(base, even, sizeof(uint64_t) * g.half, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(base + g.half, odd, sizeof(uint64_t) * g.half, hipMemcpyHostToDevice));
-  // [parity][direction][h] -> [parity][h][direction]
+  std::vector<uint64_t> w((size_t)2 * g.half);
+  spins_to_device(g, even, w.data());
+  spins_to_device(g, odd, w.data() + g.half);
+  HIP_TRY(hipMemcpy(e->t.spin + (size_t)s * 2 * g.half, w.data(), sizeof(uint64_t) * w.size(),
+                    hipMemcpyHostToDevice));
+  // [parity][direction][h] -> [parity][direction pair][h][2]
   std::vector<uint64_t> t((size_t)12 * g.n);
   for (int p = 0; p < 2; ++p)
     for (int d = 0; d < 6; ++d)
       for (int h = 0; h < g.n; ++h)
-        t[((size_t)p * g.n + h) * 6 + d] = J[((size_t)p * 6 + d) * g.n + h];
+        t[coupling_index(g, 0, p, h, d)] = J[((size_t)p * 6 + d) * g.n + h];
   HIP_TRY(hipMemcpy(const_cast<uint64_t *>(e->t.J) + (size_t)s * 12 * g.n, t.data(),
                     sizeof(uint64_t) * t.size(), hipMemcpyHostToDevice));
   e->part_pending = e->ener_valid = false;
@@ -456,9 +473,11 @@ extern "C" int ea_download_spins(ea_engine *e, int s, uint64_t *even, uint64_t *
   const Lattice &g = e->g();
   if (s < 0 || s >= g.ns) return fail(EA_E_ARG, "sample %d out of range", s);
   HIP_TRY(hipStreamSynchronize(e->stream));
-  const uint64_t *base = e->t.spin + (size_t)s * 2 * g.half;
-  HIP_TRY(hipMemcpy(even, base, sizeof(uint64_t) * g.half, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(odd, base + g.half, sizeof(uint64_t) * g.half, hipMemcpyDeviceToHost));
+  std::vector<uint64_t> w((size_t)2 * g.half);
+  HIP_TRY(hipMemcpy(w.data(), e->t.spin + (size_t)s * 2 * g.half, sizeof(uint64_t) * w.size(),
+                    hipMemcpyDeviceToHost));
+  spins_from_device(g, w.data(), even);
+  spins_from_device(g, w.data() + g.half, odd);
   return EA_OK;
 }
 
@@ -678,7 +697,22 @@ static int check_shape(const ea_shape *sh) {
   if (sh->cluster_threads != 256 && sh->cluster_threads != 512 && sh->cluster_threads != 1024)
     return fail(EA_E_ARG, "cluster_threads must be 256, 512 or 1024 (got %d)",
                 sh->cluster_threads);
+  if (sh->sweep_kind != 0 && sh->sweep_kind != 1)
+    return fail(EA_E_ARG, "sweep_kind must be 0 or 1 (got %d)", sh->sweep_kind);
+  if (sh->sweep_kind == 1 && sh->sweep_positions != 64)
+    return fail(EA_E_ARG, "sweep_kind 1 takes sweep_positions 64");
+  if (sh->sweep_xcd != 0 && sh->sweep_xcd != 1)
+    return fail(EA_E_ARG, "sweep_xcd must be 0 or 1 (got %d)", sh->sweep_xcd);
   return EA_OK;
+}
+
+static Shape to_shape(const ea_shape *sh) {
+  Shape x;
+  x.sweep_positions = sh->sweep_positions;
+  x.cluster_threads = sh->cluster_threads;
+  x.sweep_kind = sh->sweep_kind;
+  x.sweep_xcd = sh->sweep_xcd;
+  return x;
 }
 
 // Partials by sweep block change layout with the shape: pending ones are
@@ -687,9 +721,8 @@ extern "C" int ea_set_shape(ea_engine *e, const ea_shape *shape) {
   if (!e) return fail(EA_E_ARG, "NULL engine");
   RC_TRY(check_shape(shape));
   if (e->part_pending) RC_TRY(ensure_energy(e));
-  e->shape.sweep_positions = shape->sweep_positions;
-  e->shape.cluster_threads = shape->cluster_threads;
-  e->t.nbx = sweep_blocks_x(e->g(), e->shape.sweep_positions);
+  e->shape = to_shape(shape);
+  e->t.nbx = sweep_partials_x(e->g(), e->shape);
   RC_TRY(upload_tube(e));
   HIP_TRY(upload_plan(make_plan({e->t}, e->shape), &e->plan));
   return EA_OK;
@@ -875,8 +908,7 @@ static int drop_graph(ea_batch *b) {
 extern "C" int ea_batch_set_shape(ea_batch *b, const ea_shape *shape) {
   if (!b) return fail(EA_E_ARG, "NULL batch");
   RC_TRY(check_shape(shape));
-  b->launch.sweep_positions = shape->sweep_positions;
-  b->launch.cluster_threads = shape->cluster_threads;
+  b->launch = to_shape(shape);
   return EA_OK;
 }
 
@@ -901,7 +933,7 @@ extern "C" int ea_batch_run(ea_batch *b, const ea_schedule *sch, uint64_t *rng_o
     ea_engine *e = b->eng[i];
     RC_TRY(load_schedule(e, &sch[i]));
     tubes[i] = e->t;
-    tubes[i].nbx = sweep_blocks_x(e->g(), b->launch.sweep_positions);
+    tubes[i].nbx = sweep_partials_x(e->g(), b->launch);
     cluster[i] = e->t.cluster;
     sh.cluster = sh.cluster || e->t.cluster;
     sh.unfused = sh.unfused || !measured_in_pt(e->t);

@@ -102,13 +102,25 @@ struct Where {
   const Tube *t;
   int local, step;
 };
-__device__ __forceinline__ Where locate(const Tube *tubes, const uint64_t *map, int step) {
-  const uint64_t e = map ? map[blockIdx.x] : (uint64_t)blockIdx.x;
+__device__ __forceinline__ Where locate_block(const Tube *tubes, const uint64_t *map, int step,
+                                             uint32_t block) {
+  const uint64_t e = map ? map[block] : (uint64_t)block;
   Where w;
   w.t = tubes + (e >> 32);
   w.local = (int)(uint32_t)e;
   w.step = step >= 0 ? step : *w.t->step;
   return w;
+}
+__device__ __forceinline__ Where locate(const Tube *tubes, const uint64_t *map, int step) {
+  return locate_block(tubes, map, step, blockIdx.x);
+}
+
+// Workgroups are dealt to the 8 XCDs in turn (block b runs on XCD b % 8).
+// This bijection of [0, nb) gives XCD x a contiguous range of logical
+// blocks, so neighbouring blocks share an L2.
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
 // Bit permutations bringing a neighbour word whose MSC site wrapped around
@@ -195,6 +207,17 @@ __device__ __forceinline__ void load_neighbours(const Lattice &g,
   }
 }
 
+// The six couplings of half site h of parity p of sample s.
+__device__ __forceinline__ void load_couplings(const Lattice &g, const uint64_t *J, int s, int p,
+                                               int h, uint64_t out[6]) {
+  const uint64_t *j = J + coupling_index(g, s, p, h, 0);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    out[2 * k] = j[(size_t)k * 2 * g.n];
+    out[2 * k + 1] = j[(size_t)k * 2 * g.n + 1];
+  }
+}
+
 // Sum of e over the lanes of each `width`-lane group of this wave holding
 // the same replica (lane & (nr-1)); result valid in every lane.
 __device__ __forceinline__ int replica_sum(int e, int nr, int width = 64) {
@@ -204,113 +227,330 @@ __device__ __forceinline__ int replica_sum(int e, int nr, int width = 64) {
 
 // ------------------------------------------------------------------ sweep
 
-// One half-sweep of every sample.  A block of 1024 threads owns POS (64 or
-// 32) consecutive (h, r) positions of one beta.  First 6*POS of its threads
-// each make one of the six random words of the reference thread
-// tid = beta*n*nr + h*nr + r (d_gen_rndbits_cuda, main.cu:1095-1150) with the
-// beta's LUT in LDS; then each group of POS lanes updates one sample,
-// 1024/POS samples at a time, each the clone that sits at beta in its sample
-// (d_oneMCstepBN_multisample, main.cu:1385-1510).  The random words are
-// shared by all samples, are made once and never reach HBM.
 constexpr int kSwThreads = 1024;
 
-template <int POS>
-__global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *tubes, const uint64_t *map,
-                                                      Phase ph) {
-  static_assert(POS == 32 || POS == 64, "a sample's positions stay within one wave");
-  constexpr int kSwPos = POS, kSwSlots = kSwThreads / POS;
-  __shared__ uint32_t lut[EA_LUT_BYTES / 4];
-  __shared__ uint8_t clone_at[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
-  __shared__ uint32_t words[kSwPos][6];
-  __shared__ uint64_t ids[kSwPos][2];
-  __shared__ uint32_t ctr[EA_MAX_REPLICAS][5];
-  const Where wh = locate(tubes, map, ph.step);
-  const Tube &a = *wh.t;
-  if (wh.step >= a.nsteps) return;
-  const Lattice g = a.g;
-  const int beta = wh.local / a.nbx, bxi = wh.local - beta * a.nbx, tx = threadIdx.x;
-  const uint32_t *src =
-      reinterpret_cast<const uint32_t *>(a.lut + (size_t)beta * EA_LUT_BYTES);
-  for (int i = tx; i < EA_LUT_BYTES / 4; i += kSwThreads) lut[i] = src[i];
-  for (int i = tx; i < g.ns * g.nr; i += kSwThreads)
-    clone_at[i] = a.wctb[(size_t)i * g.nt + beta];
+// The beta's LUT, the clone at beta of every (sample, replica) (with ROWS,
+// the word offset of that clone's row, spin_row), and the replicas' counter
+// blocks and keys of this half-sweep, into LDS.
+template <bool ROWS, typename C>
+__device__ __forceinline__ void sweep_tables(const Tube &a, int beta, int step, int half,
+                                             uint32_t *lut, C *clone_at, uint32_t (*ctr)[5]) {
+  const Lattice &g = a.g;
+  const int tx = threadIdx.x, T = blockDim.x;
+  const uint4 *src = reinterpret_cast<const uint4 *>(a.lut + (size_t)beta * EA_LUT_BYTES);
+  uint4 *dst = reinterpret_cast<uint4 *>(lut);
+  for (int i = tx; i < EA_LUT_BYTES / 16; i += T) dst[i] = src[i];
+  for (int i = tx; i < g.ns * g.nr; i += T) {
+    const int c = a.wctb[(size_t)i * g.nt + beta];
+    clone_at[i] = ROWS ? (C)spin_row(g, i & (g.nr - 1), c) : (C)c;
+  }
   if (tx < g.nr) {
-    const ea_counter c = a.ctr[((size_t)wh.step * a.halves + ph.half) * g.nr + tx];
+    const ea_counter c = a.ctr[((size_t)step * a.halves + half) * g.nr + tx];
     ctr[tx][0] = c.x;
     ctr[tx][1] = c.y;
     ctr[tx][2] = c.z;
     ctr[tx][3] = c.w;
     ctr[tx][4] = a.keys[tx];
   }
-  __syncthreads();
+}
 
-  const int nwords = g.n * g.nr, p0 = bxi * kSwPos;
-  if (tx < 6 * kSwPos) {
-    const int pos = tx / 6, k = tx - 6 * pos, t = p0 + pos;
-    if (t < nwords) {
-      const int r = t & (g.nr - 1);
-      const uint32_t tid = (uint32_t)beta * (uint32_t)nwords + (uint32_t)t;
-      words[pos][k] = rnd_word(ctr[r][0] + k, ctr[r][1], ctr[r][2] + tid,
-                               ctr[r][3], tid, ctr[r][4], lut);
-    }
+// Global-memory view of the tube's arrays: the table holds generic
+// pointers, and loads through them would compile to flat loads, whose waits
+// also drain LDS traffic and serialise the samples' loads.
+typedef __attribute__((address_space(1))) uint64_t gword;
+__device__ __forceinline__ gword *gmem(uint64_t *p) { return (gword *)p; }
+__device__ __forceinline__ const gword *gmem(const uint64_t *p) { return (const gword *)p; }
+
+// Sample-independent addressing of one (h, r) position of parity p: word
+// offsets of its own word and of its six neighbours within a clone row of a
+// sample's spin block, of its couplings within a sample's coupling block,
+// and the boundary fix-ups its neighbours need (main.cu:1440-1472).
+struct LaneSite {
+  uint32_t own, nb[6], jo;
+  int wrap;
+  bool top, bottom;
+};
+
+__device__ __forceinline__ LaneSite lane_site(const Lattice &g, int p, int h) {
+  const Nbrs nb = neighbours(g, p, h);
+  LaneSite ls;
+  ls.own = (uint32_t)(p * g.half) + (uint32_t)h;
+#pragma unroll
+  for (int d = 0; d < 6; ++d) ls.nb[d] = (uint32_t)((1 - p) * g.half) + (uint32_t)nb.h[d];
+  ls.jo = (uint32_t)coupling_index(g, 0, p, h, 0);
+  ls.wrap = nb.wrap;
+  ls.top = g.bc == EA_OBC && nb.gz == g.MLz - 1;
+  ls.bottom = g.bc == EA_OBC && nb.gz == 0;
+  return ls;
+}
+
+// One sample's inputs at a position: its word, the six neighbours and the
+// six couplings.
+struct SiteIn {
+  uint64_t sv, sc[6], J[6];
+};
+
+__device__ __forceinline__ void load_lane(const gword *S, const gword *Jb, uint32_t c,
+                                          uint32_t jpair, const LaneSite &ls, SiteIn &x) {
+  x.sv = S[c + ls.own];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) x.sc[d] = S[c + ls.nb[d]];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    x.J[2 * k] = Jb[ls.jo + k * jpair];
+    x.J[2 * k + 1] = Jb[ls.jo + k * jpair + 1];
+  }
+}
+
+// Neighbours brought to the centre's copy layout (load_neighbours), as
+// selects rather than branches so that no wait sits between two samples'
+// loads.
+__device__ __forceinline__ void fix_lane(const LaneSite &ls, int parity, SiteIn &x) {
+  const int w = ls.wrap;
+  x.sc[0] = (w & 1) ? wrap_xp(x.sc[0]) : x.sc[0];
+  x.sc[1] = (w & 2) ? wrap_yp(x.sc[1]) : x.sc[1];
+  x.sc[2] = (w & 4) ? wrap_zp(x.sc[2]) : x.sc[2];
+  x.sc[3] = (w & 8) ? wrap_xm(x.sc[3]) : x.sc[3];
+  x.sc[4] = (w & 16) ? wrap_ym(x.sc[4]) : x.sc[4];
+  x.sc[5] = (w & 32) ? wrap_zm(x.sc[5]) : x.sc[5];
+  const uint64_t aux = parity == 0 ? x.sc[1] : x.sc[4];
+  x.sc[2] = ls.top ? (x.sc[2] & ~kMaskD) | (aux & kMaskD) : x.sc[2];
+  x.sc[5] = ls.bottom ? (x.sc[5] & ~kMaskU) | (aux & kMaskU) : x.sc[5];
+}
+
+// v_bitop3_b32 (gfx950): bit i of the result is bit (a_i << 2 | b_i << 1 |
+// c_i) of an 8-bit table.  Tables are written as the function applied to
+// A = 0xF0, B = 0xCC, C = 0xAA.
+constexpr uint32_t kT3A = 0xF0, kT3B = 0xCC, kT3C = 0xAA;
+constexpr uint32_t kXor3 = kT3A ^ kT3B ^ kT3C;                        // a ^ b ^ c
+constexpr uint32_t kMaj3 = (kT3A & kT3B) | (kT3C & (kT3A | kT3B));    // majority
+constexpr uint32_t kAnd3 = kT3A & kT3B & kT3C;                        // a & b & c
+constexpr uint32_t kOr3 = kT3A | kT3B | kT3C;                         // a | b | c
+constexpr uint32_t kAndOr = (kT3A & kT3B) | kT3C;                     // (a & b) | c
+
+template <uint32_t T>
+__device__ __forceinline__ uint64_t op3(uint64_t a, uint64_t b, uint64_t c) {
+  const uint32_t lo =
+      __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, (T & 0xFF));
+  const uint32_t hi = __builtin_amdgcn_bitop3_b32(
+      (uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32), (T & 0xFF));
+  return (uint64_t)hi << 32 | lo;
+}
+
+// The Metropolis flips of one word: (j2 j1) = f0+f1+f2 and (j4 j3) =
+// f3+f4+f5 unsatisfied bonds, id = id1 + 2 id2; flip iff j + id >= 3, i.e.
+// j1 j3 id1 all set, or a weight-2 term together with a weight-1 term, or
+// two weight-2 terms (majority of j2, j4, id2).  With `energy` also the
+// word's bond energy after the move.
+__device__ __forceinline__ uint64_t site_flip(const SiteIn &x, uint64_t id1, uint64_t id2,
+                                              bool energy, int &e) {
+  uint64_t f[6];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) f[d] = op3<kXor3>(x.sv, x.J[d], x.sc[d]);  // 1 = unsatisfied
+  const uint64_t j1 = op3<kXor3>(f[0], f[1], f[2]), j2 = op3<kMaj3>(f[0], f[1], f[2]);
+  const uint64_t j3 = op3<kXor3>(f[3], f[4], f[5]), j4 = op3<kMaj3>(f[3], f[4], f[5]);
+  const uint64_t one = op3<kOr3>(j1, j3, id1), two = op3<kOr3>(j2, j4, id2);
+  const uint64_t flip =
+      op3<kAndOr>(one, two, op3<kMaj3>(j2, j4, id2)) | op3<kAnd3>(j1, j3, id1);
+  if (energy) {
+    int u = 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) u += __popcll(f[d] ^ flip);
+    e = 6 * 64 - 2 * u;
+  }
+  return flip;
+}
+
+// Sweep block order: a block holds consecutive half sites of one replica
+// (a contiguous run of each clone row it touches), and the launch order is
+// site range, then replica, then beta, so that the blocks reading the same
+// couplings run together and share them in L2.
+struct SweepBlock {
+  int hb, r, beta;
+};
+__device__ __forceinline__ SweepBlock sweep_block(const Lattice &g, int local) {
+  SweepBlock b;
+  const int rest = local / g.nt;
+  b.beta = local - rest * g.nt;
+  b.hb = rest >> g.nr_shift;
+  b.r = rest & (g.nr - 1);
+  return b;
+}
+
+// The six random words of position (h, r) at beta, as the reference thread
+// tid = beta*n*nr + h*nr + r makes them (d_gen_rndbits_cuda,
+// main.cu:1095-1150), folded into the Metropolis thresholds id = id1 + 2 id2
+// with P(id >= k) = exp(-4 k beta).
+__device__ __forceinline__ uint32_t sweep_word(const Lattice &g, const uint32_t (*ctr)[5],
+                                               const uint32_t *lut, int beta, int h, int r,
+                                               int k) {
+  const uint32_t tid = ((uint32_t)beta * (uint32_t)g.n + (uint32_t)h) * (uint32_t)g.nr + r;
+  return rnd_word(ctr[r][0] + k, ctr[r][1], ctr[r][2] + tid, ctr[r][3], tid, ctr[r][4], lut);
+}
+__device__ __forceinline__ void thresholds(const uint32_t w[6], uint64_t &id1, uint64_t &id2) {
+  const uint64_t b0 = ((uint64_t)w[0] << 32) | w[1];
+  const uint64_t b1 = (((uint64_t)w[2] << 32) | w[3]) & b0;
+  const uint64_t b2 = (((uint64_t)w[4] << 32) | w[5]) & b1;
+  id1 = (b0 ^ b1) | b2;
+  id2 = b1;
+}
+
+// One sample's update at a position: loads, boundary fix-ups, flips and
+// the store, two samples at a time so that both samples' loads are in
+// flight together.  Returns the words' energies after the move in e0, e1
+// when `energy`.
+__device__ __forceinline__ void update_two(uint64_t *spin, const uint64_t *J, size_t sstride,
+                                           size_t jstride, uint32_t jpair, int s0, int s1,
+                                           uint32_t c0, uint32_t c1, const LaneSite &ls, int p,
+                                           uint64_t id1, uint64_t id2, bool energy, int &e0,
+                                           int &e1) {
+  gword *S0 = gmem(spin + s0 * sstride), *S1 = gmem(spin + s1 * sstride);
+  SiteIn x0, x1;
+  load_lane(S0, gmem(J + s0 * jstride), c0, jpair, ls, x0);
+  load_lane(S1, gmem(J + s1 * jstride), c1, jpair, ls, x1);
+  fix_lane(ls, p, x0);
+  fix_lane(ls, p, x1);
+  const uint64_t f0 = site_flip(x0, id1, id2, energy, e0);
+  const uint64_t f1 = site_flip(x1, id1, id2, energy, e1);
+  S0[c0 + ls.own] = x0.sv ^ f0;
+  if (s1 != s0) S1[c1 + ls.own] = x1.sv ^ f1;
+}
+
+// Sum over the first `width` lanes of each width-lane group.
+template <int W>
+__device__ __forceinline__ int group_sum(int e) {
+#pragma unroll
+  for (int off = 1; off < W; off <<= 1) e += __shfl_xor(e, off, 64);
+  return e;
+}
+
+// One half-sweep (d_oneMCstepBN_multisample, main.cu:1385-1510), kind 0: a
+// block of 1024 threads owns POS consecutive half sites of one replica at
+// one beta.  First 6*POS threads each make one of the six random words of a
+// position with the beta's LUT in LDS; then each group of POS lanes
+// updates one sample's clone at beta, 1024/POS samples at a time.  The
+// random words are shared by all samples, are made once and never reach
+// HBM.  One energy partial per (sample, replica, beta, block).
+template <int POS, bool XCD>
+__global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *tubes, const uint64_t *map,
+                                                      Phase ph) {
+  static_assert(POS == 32 || POS == 64, "a sample's positions stay within one wave");
+  constexpr int kSlots = kSwThreads / POS;
+  __shared__ __align__(16) uint32_t lut[EA_LUT_BYTES / 4];
+  __shared__ uint32_t clone_off[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
+  __shared__ uint32_t words[POS][6];
+  __shared__ uint64_t ids[POS][2];
+  __shared__ uint32_t ctr[EA_MAX_REPLICAS][5];
+  const Where wh = locate_block(tubes, map, ph.step,
+                                XCD ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const SweepBlock b = sweep_block(g, wh.local);
+  const int tx = threadIdx.x, h0 = b.hb * POS, r = b.r;
+  sweep_tables<true>(a, b.beta, wh.step, ph.half, lut, clone_off, ctr);
+  __syncthreads();
+  if (tx < 6 * POS) {
+    const int pos = tx / 6, k = tx - 6 * pos;
+    if (h0 + pos < g.n) words[pos][k] = sweep_word(g, ctr, lut, b.beta, h0 + pos, r, k);
   }
   __syncthreads();
-  if (tx < kSwPos) {
-    const uint32_t *w = words[tx];
-    const uint64_t b0 = ((uint64_t)w[0] << 32) | w[1];
-    const uint64_t b1 = (((uint64_t)w[2] << 32) | w[3]) & b0;
-    const uint64_t b2 = (((uint64_t)w[4] << 32) | w[5]) & b1;
-    // id = id1 + 2 id2 in 0..3 with P(id >= k) = exp(-4 k beta)
-    ids[tx][0] = (b0 ^ b1) | b2;
-    ids[tx][1] = b1;
-  }
+  if (tx < POS) thresholds(words[tx], ids[tx][0], ids[tx][1]);
   __syncthreads();
 
-  const int pos = tx & (kSwPos - 1), slot = tx / kSwPos;
-  const int t = p0 + pos;
-  const bool valid = t < nwords;
-  const int h = valid ? t >> g.nr_shift : 0, r = t & (g.nr - 1);
+  const int pos = tx & (POS - 1), slot = tx / POS;
+  const bool valid = h0 + pos < g.n;
+  const int h = valid ? h0 + pos : 0;
   const uint64_t id1 = ids[pos][0], id2 = ids[pos][1];
   const int p = ph.half & 1;
-  const Nbrs nb = neighbours(g, p, h);
-
-  for (int s = slot; s < g.ns; s += kSwSlots) {
-    const int clone = clone_at[s * g.nr + r];
-    int e = 0;
-    if (valid) {
-      uint64_t *base = a.spin + (size_t)s * 2 * g.half + (size_t)clone * nwords;
-      uint64_t *cur = base + (size_t)p * g.half;
-      const uint64_t *old = base + (size_t)(1 - p) * g.half;
-      const uint64_t sv = cur[(size_t)h * g.nr + r];
-      uint64_t sc[6];
-      load_neighbours(g, old, r, nb, p, sc, g.nr);
-      const uint64_t *Jw = a.J + ((size_t)(2 * s + p) * g.n + h) * 6;
-      uint64_t f[6];
-#pragma unroll
-      for (int d = 0; d < 6; ++d) f[d] = sv ^ Jw[d] ^ sc[d];  // 1 = unsatisfied
-      // (j2 j1) = f0+f1+f2, (j4 j3) = f3+f4+f5; flip iff j + id >= 3
-      uint64_t k1 = f[0] ^ f[1], k2 = f[0] & f[1];
-      const uint64_t j1 = k1 ^ f[2], j2 = k2 ^ (k1 & f[2]);
-      k1 = f[3] ^ f[4];
-      k2 = f[3] & f[4];
-      const uint64_t j3 = k1 ^ f[5], j4 = k2 ^ (k1 & f[5]);
-      const uint64_t j24 = j2 | j4;
-      const uint64_t flip = (j1 & j3 & id1) | ((j24 | id2) & (j1 | j3 | id1)) |
-                            (j2 & j4) | (j24 & id2);
-      cur[(size_t)h * g.nr + r] = sv ^ flip;
-      if (ph.energy) {
-        int u = 0;
-#pragma unroll
-        for (int d = 0; d < 6; ++d) u += __popcll(f[d] ^ flip);
-        e = 6 * 64 - 2 * u;
-      }
-    }
+  const LaneSite ls = lane_site(g, p, h);
+  uint64_t *const spin = a.spin;
+  const uint64_t *const J = a.J;
+  int32_t *const part = a.part;
+  const int nbx = a.nbx;
+  const size_t sstride = 2 * (size_t)g.half, jstride = 12 * (size_t)g.n;
+  const uint32_t jpair = 2 * (uint32_t)g.n;
+  const uint32_t pstride = (uint32_t)(g.nr * g.nt * nbx);  // partials of one sample
+  const uint32_t pbase = (uint32_t)((r * g.nt + b.beta) * nbx + b.hb);
+  for (int s0 = slot; s0 < g.ns; s0 += 2 * kSlots) {
+    const int s1 = s0 + kSlots < g.ns ? s0 + kSlots : s0;
+    int e0 = 0, e1 = 0;
+    if (valid)
+      update_two(spin, J, sstride, jstride, jpair, s0, s1, clone_off[s0 * g.nr + r],
+                 clone_off[s1 * g.nr + r], ls, p, id1, id2, ph.energy, e0, e1);
     if (ph.energy) {
       // every bond has exactly one odd end: the odd half-sweep sees them
       // all; one partial per (sample, replica, beta, block), no atomics
-      e = replica_sum(e, g.nr, kSwPos);
-      if (pos < g.nr)
-        a.part[(((size_t)s * g.nr + r) * g.nt + beta) * a.nbx + bxi] = e;
+      e0 = group_sum<POS>(e0);
+      e1 = group_sum<POS>(e1);
+      if (pos == 0) {
+        part[(uint32_t)s0 * pstride + pbase] = e0;
+        if (s1 != s0) part[(uint32_t)s1 * pstride + pbase] = e1;
+      }
+    }
+  }
+}
+
+// One half-sweep, kind 1: one thread per half site of one replica at one
+// beta makes the position's six random words itself and updates every
+// sample's clone at beta, two samples' loads in flight at a time.  Blocks of
+// 256 threads (256 consecutive half sites), one energy partial per wave.
+constexpr int kLaneThreads = 256;
+
+template <bool XCD>
+__global__ void __launch_bounds__(kLaneThreads) k_sweep_lane(const Tube *tubes,
+                                                             const uint64_t *map, Phase ph) {
+  __shared__ __align__(16) uint32_t lut[EA_LUT_BYTES / 4];
+  __shared__ uint32_t clone_off[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
+  __shared__ uint32_t ctr[EA_MAX_REPLICAS][5];
+  const Where wh = locate_block(tubes, map, ph.step,
+                                XCD ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const SweepBlock b = sweep_block(g, wh.local);
+  const int tx = threadIdx.x, r = b.r;
+  sweep_tables<true>(a, b.beta, wh.step, ph.half, lut, clone_off, ctr);
+  __syncthreads();
+
+  const int hh = b.hb * kLaneThreads + tx;
+  const bool valid = hh < g.n;
+  const int h = valid ? hh : 0;
+  uint64_t id1 = 0, id2 = 0;
+  if (valid) {
+    uint32_t w[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = sweep_word(g, ctr, lut, b.beta, h, r, k);
+    thresholds(w, id1, id2);
+  }
+  const int p = ph.half & 1;
+  const LaneSite ls = lane_site(g, p, h);
+  // the tube's pointers stay in registers: stores through them could
+  // otherwise alias the descriptor and force reloads
+  uint64_t *const spin = a.spin;
+  const uint64_t *const J = a.J;
+  int32_t *const part = a.part;
+  const int nbx = a.nbx, bx = b.hb * (kLaneThreads / 64) + (tx >> 6);
+  const uint32_t pstride = (uint32_t)(g.nr * g.nt * nbx);  // partials of one sample
+  const uint32_t pbase = (uint32_t)((r * g.nt + b.beta) * nbx + bx);
+  const bool pwrite = (tx & 63) == 0 && bx < nbx;  // a wave past the last site has none
+  const size_t sstride = 2 * (size_t)g.half, jstride = 12 * (size_t)g.n;
+  const uint32_t jpair = 2 * (uint32_t)g.n;  // words between direction pairs
+  for (int s0 = 0; s0 < g.ns; s0 += 2) {
+    const int s1 = s0 + 1 < g.ns ? s0 + 1 : s0;
+    int e0 = 0, e1 = 0;
+    if (valid)
+      update_two(spin, J, sstride, jstride, jpair, s0, s1, clone_off[s0 * g.nr + r],
+                 clone_off[s1 * g.nr + r], ls, p, id1, id2, ph.energy, e0, e1);
+    if (ph.energy) {
+      // every bond has exactly one odd end: one partial per (sample,
+      // replica, beta, 64 sites), no atomics
+      e0 = group_sum<64>(e0);
+      e1 = group_sum<64>(e1);
+      if (pwrite) {
+        part[(uint32_t)s0 * pstride + pbase] = e0;
+        if (s1 != s0) part[(uint32_t)s1 * pstride + pbase] = e1;
+      }
     }
   }
 }
@@ -340,12 +580,12 @@ __global__ void __launch_bounds__(256) k_energy(const Tube *tubes, const uint64_
   const int h = valid ? t >> g.nr_shift : 0, r = t & (g.nr - 1);
   int e = 0;
   if (valid) {
-    const uint64_t *base = a.spin + (size_t)s * 2 * g.half + (size_t)clone * nwords;
-    const uint64_t sv = base[(size_t)h * g.nr + r];
+    const uint64_t *base = a.spin + (size_t)s * 2 * g.half + spin_row(g, r, clone);
+    const uint64_t sv = base[h];
     const Nbrs nb = neighbours(g, 0, h);
-    uint64_t sc[6];
-    load_neighbours(g, base + g.half, r, nb, 0, sc, g.nr);
-    const uint64_t *Jw = a.J + ((size_t)(2 * s) * g.n + h) * 6;
+    uint64_t sc[6], Jw[6];
+    load_neighbours(g, base + g.half, 0, nb, 0, sc, 1);
+    load_couplings(g, a.J, s, 0, h, Jw);
     int u = 0;
 #pragma unroll
     for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
@@ -626,8 +866,8 @@ __device__ __forceinline__ Pair pair_words(const Lattice &g, const Tube &a, cons
   const int c2 = a.wctb[((size_t)s * g.nr + r2) * g.nt + beta];
   const size_t sb = (size_t)s * 2 * g.half;
   Pair q;
-  q.o1 = sb + (size_t)c1 * g.n * g.nr + r1;
-  q.o2 = sb + (size_t)c2 * g.n * g.nr + r2;
+  q.o1 = sb + spin_row(g, r1, c1);
+  q.o2 = sb + spin_row(g, r2, c2);
   return q;
 }
 
@@ -886,7 +1126,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uin
   if (tid < 2) esum[tid] = 0;
   for (int w = tid; w < 2 * g.n; w += T) {
     const int p = w >= g.n, h = w - p * g.n;
-    const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+    const size_t o = (size_t)p * g.half + h;
     qw[w] = a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
   }
   for (int i = tid; i < V; i += T) f.p[i] = i;
@@ -971,7 +1211,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uin
     m |= __shfl_xor(m, 1, 64);
     m |= __shfl_xor(m, 2, 64);
     if (live && quarter == 0) {
-      const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+      const size_t o = (size_t)p * g.half + h;
       const uint64_t qn = qw[w] ^ m;  // q' = q ^ coin(label)
       const uint64_t x1 = a.spin[pw.o1 + o] ^ qn, x2 = a.spin[pw.o2 + o] ^ qn;
       a.spin[pw.o1 + o] = x1;
@@ -992,7 +1232,8 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uin
       uint64_t sc[6];
       load_neighbours(g, wd + g.n, 0, nb, 0, sc, 1);
       const uint64_t sv = wd[h];
-      const uint64_t *Jw = a.J + ((size_t)(2 * s) * g.n + h) * 6;
+      uint64_t Jw[6];
+      load_couplings(g, a.J, s, 0, h, Jw);
       int u = 0;
 #pragma unroll
       for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
@@ -1032,7 +1273,7 @@ __device__ __forceinline__ uint32_t q_site(const Lattice &g, const Tube &a,
   const int bx = qdiv(x, g.mML), by = qdiv(y, g.mML), bz = qdiv(z, g.mMLz);
   const int bit = bx + 4 * by + 16 * bz;
   const int h = ((x - bx * g.ML) + g.ML * (y - by * g.ML) + g.MS * (z - bz * g.MLz)) >> 1;
-  const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+  const size_t o = (size_t)p * g.half + h;
   return (uint32_t)(((a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o]) >> bit) & 1);
 }
 
@@ -1064,7 +1305,7 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *tubes, const uint
   const int gbase = z0 * S;
   for (int r = tid; r < planes * g.L; r += T)
     rows[r] = row_bits(g, z0 * g.L + r, [&](int p, int h) {
-      const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+      const size_t o = (size_t)p * g.half + h;
       return a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
     });
   const uint64_t cidx = in.coin + (uint64_t)in.s * g.V * g.nt + (uint64_t)in.beta * g.V + gbase;
@@ -1135,7 +1376,7 @@ __global__ void __launch_bounds__(256) k_hou_apply(const Tube *tubes, const uint
   const uint32_t *cb = a.hcb + ib * (g.V / 32);
   const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
   const int p = w >= g.n, h = w - p * g.n;
-  const size_t o = (size_t)p * g.half + (size_t)h * g.nr;
+  const size_t o = (size_t)p * g.half + h;
   const int base = site_base(g, p, h);
   uint64_t m = 0;
   for (int b = 0; b < 64; ++b) {
@@ -1154,15 +1395,21 @@ inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) /
 
 // ------------------------------------------------------------- launchers
 
-int sweep_blocks_x(const Lattice &g, int positions) {
-  return (int)blocks_for((long long)g.n * g.nr, positions);
+int sweep_partials_x(const Lattice &g, const Shape &shape) {
+  return (int)blocks_for(g.n, shape.sweep_kind ? 64 : shape.sweep_positions);
+}
+
+static long long sweep_grid(const Tube &t, const Shape &shape) {
+  const Lattice &g = t.g;
+  const long long per = shape.sweep_kind ? blocks_for(g.n, kLaneThreads) : t.nbx;
+  return per * g.nr * g.nt;
 }
 
 static int energy_blocks_host(const Lattice &g) { return (int)blocks_for((long long)g.n * g.nr, 256); }
 
 size_t energy_partials(const Lattice &g) {
   const size_t per = (size_t)g.ns * g.nr * g.nt;
-  return per * std::max<size_t>(sweep_blocks_x(g, kSweepPosMin),
+  return per * std::max<size_t>(blocks_for(g.n, kSweepPosMin),
                                 4 * (size_t)energy_blocks_host(g));
 }
 
@@ -1186,7 +1433,7 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
     auto add = [&](int kind, long long count) {
       for (long long i = 0; i < count; ++i) p.map[kind].push_back(tag | (uint64_t)i);
     };
-    add(kMapSweep, (long long)t.nbx * g.nt);
+    add(kMapSweep, sweep_grid(t, shape));
     add(kMapEnergy, (long long)energy_blocks_host(g) * g.nt * g.ns);
     add(kMapChain, (long long)g.ns * g.nr);
     if (!t.cluster || g.ctm <= 0) continue;
@@ -1245,13 +1492,45 @@ void free_plan(DevPlan *d) {
   *d = DevPlan{};
 }
 
+static void allow_lds(const void *kernel, size_t bytes) {
+  // raise the dynamic LDS cap of a kernel once per process, to the largest
+  // size requested so far
+  static std::mutex mu;
+  static std::vector<std::pair<const void *, size_t>> done;
+  std::lock_guard<std::mutex> lock(mu);
+  for (auto &d : done)
+    if (d.first == kernel) {
+      if (d.second >= bytes) return;
+      d.second = bytes;
+      (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes);
+      return;
+    }
+  done.push_back({kernel, bytes});
+  (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s) {
   if (!p.cnt[kMapSweep]) return hipGetLastError();
   const dim3 grid((unsigned)p.cnt[kMapSweep]);
-  if (p.shape.sweep_positions == 32)
-    hipLaunchKernelGGL(k_sweep<32>, grid, dim3(kSwThreads), 0, s, tubes, p.map(kMapSweep), ph);
-  else
-    hipLaunchKernelGGL(k_sweep<64>, grid, dim3(kSwThreads), 0, s, tubes, p.map(kMapSweep), ph);
+  const uint64_t *m = p.map(kMapSweep);
+  const Shape &sh = p.shape;
+  if (sh.sweep_kind == 1) {
+    if (sh.sweep_xcd)
+      hipLaunchKernelGGL(k_sweep_lane<true>, grid, dim3(kLaneThreads), 0, s, tubes, m, ph);
+    else
+      hipLaunchKernelGGL(k_sweep_lane<false>, grid, dim3(kLaneThreads), 0, s, tubes, m, ph);
+  } else if (sh.sweep_positions == 32) {
+    if (sh.sweep_xcd)
+      hipLaunchKernelGGL((k_sweep<32, true>), grid, dim3(kSwThreads), 0, s, tubes, m, ph);
+    else
+      hipLaunchKernelGGL((k_sweep<32, false>), grid, dim3(kSwThreads), 0, s, tubes, m, ph);
+  } else {
+    if (sh.sweep_xcd)
+      hipLaunchKernelGGL((k_sweep<64, true>), grid, dim3(kSwThreads), 0, s, tubes, m, ph);
+    else
+      hipLaunchKernelGGL((k_sweep<64, false>), grid, dim3(kSwThreads), 0, s, tubes, m, ph);
+  }
   return hipGetLastError();
 }
 
@@ -1301,23 +1580,6 @@ hipError_t launch_step_end(const Tube *tubes, int ntubes, hipStream_t s) {
   return hipGetLastError();
 }
 
-static void allow_lds(const void *kernel, size_t bytes) {
-  // raise the dynamic LDS cap of a kernel once per process, to the largest
-  // size requested so far
-  static std::mutex mu;
-  static std::vector<std::pair<const void *, size_t>> done;
-  std::lock_guard<std::mutex> lock(mu);
-  for (auto &d : done)
-    if (d.first == kernel) {
-      if (d.second >= bytes) return;
-      d.second = bytes;
-      (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)bytes);
-      return;
-    }
-  done.push_back({kernel, bytes});
-  (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
 
 hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s) {
   if (p.cnt[kMapWhole32]) {

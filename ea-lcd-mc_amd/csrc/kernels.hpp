@@ -28,6 +28,20 @@ struct Lattice {
   uint32_t mL, mML, mMLz, mhl, mhs;
 };
 
+// Device layouts (host transfers convert to and from the reference's
+// [parity][clone][h][replica] spin words and [parity][direction][h]
+// couplings):
+//   spins of sample s: [2 parities][nr replicas][nt clones][n] words, so the
+//     words a replica's clone at one beta needs are contiguous;
+//   couplings of sample s: [2 parities][3 direction pairs][n][2] words, the
+//     two directions 2k, 2k+1 of a half site side by side.
+inline __host__ __device__ size_t spin_row(const Lattice &g, int r, int clone) {
+  return ((size_t)r * g.nt + clone) * g.n;
+}
+inline __host__ __device__ size_t coupling_index(const Lattice &g, int s, int p, int h, int d) {
+  return ((((size_t)s * 2 + p) * 3 + (d >> 1)) * g.n + h) * 2 + (d & 1);
+}
+
 inline uint32_t divisor_magic(uint32_t d) {
   return d <= 1 ? 0u : (uint32_t)(((1ULL << 32) + d - 1) / d);
 }
@@ -100,13 +114,20 @@ inline __host__ __device__ bool measured_in_pt(const Tube &t) {
   return !t.cluster || t.planes == 0;
 }
 // Launch shape of a table's kernels (chosen by measurement, ea_set_shape /
-// ea_batch_set_shape): sweep positions per 1024-thread block (32 or 64) and
-// threads of a whole-tube Houdayer workgroup (256, 512 or 1024).
+// ea_batch_set_shape).  Sweep: kind 0 shares each position's random words
+// among the samples through LDS (1024-thread blocks of sweep_positions = 32
+// or 64 half sites of one replica), kind 1 gives each thread one position
+// and all samples (256-thread blocks, one energy partial per wave);
+// sweep_xcd = 1 hands each XCD a contiguous range of sweep blocks.  cluster_threads: threads of
+// a whole-tube Houdayer workgroup (256, 512 or 1024).
 struct Shape {
   int sweep_positions = 64;
   int cluster_threads = 1024;
+  int sweep_kind = 0;
+  int sweep_xcd = 0;
   bool operator==(const Shape &o) const {
-    return sweep_positions == o.sweep_positions && cluster_threads == o.cluster_threads;
+    return sweep_positions == o.sweep_positions && cluster_threads == o.cluster_threads &&
+           sweep_kind == o.sweep_kind && sweep_xcd == o.sweep_xcd;
   }
 };
 constexpr int kSweepPosMin = 32;
@@ -132,14 +153,14 @@ struct DevPlan {
   bool ready() const { return identity || buf; }
 };
 
-// Sweep blocks per beta for a block of `positions` positions, and the
-// partial-energy slots a tube needs under any shape.
-int sweep_blocks_x(const Lattice &g, int positions);
+// Energy partials per (sample, replica, beta) a sweep of this shape writes
+// (a tube's nbx), and the partial-energy slots a tube needs under any shape.
+int sweep_partials_x(const Lattice &g, const Shape &shape);
 size_t energy_partials(const Lattice &g);
 
 // Block maps of the tubes (host copies of the descriptors give geometry,
 // Houdayer path and whether the tube has cluster moves).
-// The tubes' nbx must be sweep_blocks_x(g, shape.sweep_positions).
+// The tubes' nbx must be sweep_partials_x(g, shape).
 Plan make_plan(const std::vector<Tube> &tubes, Shape shape);
 hipError_t upload_plan(const Plan &p, DevPlan *d);
 void free_plan(DevPlan *d);

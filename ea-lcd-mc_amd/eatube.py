@@ -56,7 +56,8 @@ class Schedule(C.Structure):
 
 
 class Shape(C.Structure):
-    _fields_ = [("sweep_positions", C.c_int32), ("cluster_threads", C.c_int32)]
+    _fields_ = [("sweep_positions", C.c_int32), ("cluster_threads", C.c_int32),
+                ("sweep_kind", C.c_int32), ("sweep_xcd", C.c_int32)]
 
 
 assert C.sizeof(HQ) == 2080 and C.sizeof(XO) == 48
@@ -351,9 +352,10 @@ class Engine:
         check(lib().ea_run(self.h, C.byref(sched), C.byref(off)))
         return off.value
 
-    def set_shape(self, sweep_positions=64, cluster_threads=1024):
+    def set_shape(self, sweep_positions=64, cluster_threads=1024, sweep_kind=0,
+                  sweep_xcd=0):
         """Launch shape of this engine's calls (results do not change)."""
-        sh = Shape(sweep_positions, cluster_threads)
+        sh = Shape(sweep_positions, cluster_threads, sweep_kind, sweep_xcd)
         check(lib().ea_set_shape(self.h, C.byref(sh)))
 
     def set_timing(self, on=True):
@@ -392,9 +394,10 @@ class Batch:
         check(lib().ea_batch_run(self.h, arr, _ptr(off, C.c_uint64)))
         return off
 
-    def set_shape(self, sweep_positions=64, cluster_threads=1024):
+    def set_shape(self, sweep_positions=64, cluster_threads=1024, sweep_kind=0,
+                  sweep_xcd=0):
         """Launch shape of the batch's steps, from the next run on."""
-        sh = Shape(sweep_positions, cluster_threads)
+        sh = Shape(sweep_positions, cluster_threads, sweep_kind, sweep_xcd)
         check(lib().ea_batch_set_shape(self.h, C.byref(sh)))
 
     def close(self):
@@ -509,9 +512,15 @@ class Simulation:
 
 # -------------------------------------------------------------- autotuner
 
-# Launch shapes the engine takes: (sweep positions per block, threads of a
-# whole-tube Houdayer workgroup).
-SHAPES = ((64, 1024), (32, 1024), (64, 512), (32, 512), (64, 256), (32, 256))
+# Launch shapes the engine takes, as (sweep_positions, cluster_threads,
+# sweep_kind, sweep_xcd): the sweep options, and the Houdayer workgroup
+# sizes.
+SWEEP_OPTIONS = ((64, 0, 0), (32, 0, 0), (64, 0, 1), (32, 0, 1), (64, 1, 0),
+                 (64, 1, 1))
+CLUSTER_THREADS = (1024, 512, 256)
+DEFAULT_SHAPE = (64, 1024, 0, 0)
+SHAPES = tuple((p, c, k, x) for (p, k, x) in SWEEP_OPTIONS
+               for c in CLUSTER_THREADS)
 
 
 def tube_key(engine):
@@ -542,15 +551,17 @@ def save_shape_cache(path, cache):
 
 
 def tune_shape(engines, ptfr=1, houfr=1, mesfr=1, cluster=True, steps=8,
-               shapes=SHAPES, cache=None):
+               cache=None):
     """Fastest launch shape for advancing `engines` together, by timing.
 
     The candidates run on scratch twins (engines of the same configurations,
     betas and LUTs, with samples of their own), so the engines themselves
-    are not touched: each shape gets one untimed step (plan and graph are
-    rebuilt) and then `steps` timed steps of one batch of the twins.
-    Returns ((sweep_positions, cluster_threads), {shape: seconds}); with a
-    `cache` dict holding the mix's signature, returns its entry without
+    are not touched.  The sweep options are timed first with the default
+    Houdayer workgroup, then (with cluster moves) the workgroup sizes with
+    the best sweep option: each candidate gets one untimed step (plan and
+    graph are rebuilt) and then `steps` timed steps of one batch of the
+    twins.  Returns (shape, {shape: seconds}) with shapes as in SHAPES; with
+    a `cache` dict holding the mix's signature, returns its entry without
     timing anything, and otherwise records the winner there."""
     key = shape_signature(engines, ptfr, houfr, mesfr, cluster)
     if cache is not None and key in cache:
@@ -562,14 +573,23 @@ def tune_shape(engines, ptfr=1, houfr=1, mesfr=1, cluster=True, steps=8,
     batch = Batch([t.engine for t in twins])
     kw = dict(ptfr=ptfr, houfr=houfr, mesfr=mesfr, cluster=cluster)
     times = {}
+
+    def timed(shp):
+        batch.set_shape(*shp)
+        batch.run([t.schedule(1, **kw) for t in twins])
+        sch = [t.schedule(steps, **kw) for t in twins]
+        t0 = time.perf_counter()
+        batch.run(sch)
+        times[shp] = time.perf_counter() - t0
+
     try:
-        for shp in shapes:
-            batch.set_shape(*shp)
-            batch.run([t.schedule(1, **kw) for t in twins])
-            sch = [t.schedule(steps, **kw) for t in twins]
-            t0 = time.perf_counter()
-            batch.run(sch)
-            times[tuple(shp)] = time.perf_counter() - t0
+        for (p, k, x) in SWEEP_OPTIONS:
+            timed((p, DEFAULT_SHAPE[1], k, x))
+        p, _, k, x = min(times, key=times.get)
+        if cluster:
+            for c in CLUSTER_THREADS:
+                if (p, c, k, x) not in times:
+                    timed((p, c, k, x))
     finally:
         batch.close()
         for t in twins:

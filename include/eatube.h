@@ -234,15 +234,24 @@ int ea_batch_destroy(ea_batch *b);
 int ea_batch_run(ea_batch *b, const ea_schedule *schedules, uint64_t *rng_offsets_out);
 
 /* Launch shape (no counterpart in the reference, whose block sizes are
- * compile-time constants, main.cu:1095-1510): positions per 1024-thread
- * sweep block (32 or 64) and threads of a whole-tube Houdayer workgroup
- * (256, 512 or 1024).  Results do not depend on it, only speed does; the
- * best shape of a tube mix is found by timing (eatube.tune_shape).  The
- * default is {64, 1024}.  ea_set_shape applies to an engine's own calls,
- * ea_batch_set_shape to the batch's launches. */
+ * compile-time constants, main.cu:1095-1510).  Results do not depend on it,
+ * only speed does; the best shape of a tube mix is found by timing
+ * (eatube.tune_shape).
+ *   sweep_kind 0: each position's random words are made once and shared by
+ *     the samples through LDS, sweep_positions (32 or 64) positions per
+ *     1024-thread block;
+ *   sweep_kind 1: one thread per position updates every sample
+ *     (sweep_positions must be 64);
+ *   sweep_xcd 1: each XCD runs a contiguous range of sweep blocks;
+ *   cluster_threads: threads of a whole-tube Houdayer workgroup (256, 512
+ *     or 1024).
+ * The default is {64, 1024, 0, 0}.  ea_set_shape applies to an engine's own
+ * calls, ea_batch_set_shape to the batch's launches. */
 typedef struct ea_shape {
   int32_t sweep_positions;
   int32_t cluster_threads;
+  int32_t sweep_kind;
+  int32_t sweep_xcd;
 } ea_shape;
 int ea_set_shape(ea_engine *e, const ea_shape *shape);
 int ea_batch_set_shape(ea_batch *b, const ea_shape *shape);

@@ -133,7 +133,7 @@ def test_every_launch_shape_gives_the_same_state():
         solo.engine.set_shape(*shp)
         solo.steps(1, cluster=True)
         solo.engine.sweep(solo.counters(2))      # leaves partials pending
-        other = (32 if shp[0] == 64 else 64, shp[1])
+        other = (64, shp[1], 1 - shp[2], 1 - shp[3])
         solo.engine.set_shape(*other)            # across a pending sweep
         solo.engine.pt()
         p = make(specs[1], 31)
@@ -151,7 +151,8 @@ def test_tuner_picks_a_shape_and_caches_it(tmp_path):
     best, times = eatube.tune_shape([s.engine for s in sims], steps=2,
                                     cache=cache)
     say("tuned", best, times)
-    assert best in eatube.SHAPES and set(times) == set(eatube.SHAPES)
+    assert best in eatube.SHAPES and best in times
+    assert set(times) <= set(eatube.SHAPES) and len(times) >= 6
     assert min(times.values()) == times[best]
     for b, s in zip(before, sims):     # the engines themselves are untouched
         same(b, state(s), "after tuning")

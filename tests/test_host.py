@@ -39,7 +39,7 @@ def test_errors_are_reported_not_fatal():
     rc = eatube.lib().ea_create(C.byref(c), C.byref(h))
     assert rc == -1
     assert b"multiples of 8" in eatube.lib().ea_last_error()
-    sh = eatube.Shape(64, 1024)
+    sh = eatube.Shape(64, 1024, 0, 0)
     assert eatube.lib().ea_set_shape(None, C.byref(sh)) == -1
     assert eatube.lib().ea_batch_set_shape(None, C.byref(sh)) == -1
 

@@ -132,5 +132,7 @@ def test_shape_signature_and_cache(tmp_path):
     eatube.save_shape_cache(path, {"k": [32, 512]})
     assert eatube.load_shape_cache(path) == {"k": [32, 512]}
     # a cached mix is answered without building anything
-    cache = {eatube.shape_signature([a, b]): [32, 512]}
-    assert eatube.tune_shape([b, a], cache=cache) == ((32, 512), {})
+    cache = {eatube.shape_signature([a, b]): [32, 512, 0, 1]}
+    assert eatube.tune_shape([b, a], cache=cache) == ((32, 512, 0, 1), {})
+    assert eatube.DEFAULT_SHAPE in eatube.SHAPES
+    assert len(set(eatube.SHAPES)) == len(eatube.SHAPES) == 18
