@@ -250,6 +250,9 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape")
+    ap.add_argument("--shape", default=None,
+                    help="launch shape P,C,K,X (sweep positions, cluster "
+                         "threads, sweep kind, XCD order) instead of tuning")
     ap.add_argument("--timing-steps", type=int, default=50)
     ap.add_argument("--batch-steps", type=int, default=200,
                     help="steps of the 48-tube mixed batch comparison (0: skip)")
@@ -290,7 +293,9 @@ def main():
     eng = sim.engine
     ctm = eng.ctm
     shape, shape_times = eatube.DEFAULT_SHAPE, {}
-    if not args.no_tune:
+    if args.shape:
+        shape = tuple(int(x) for x in args.shape.split(","))
+    elif not args.no_tune:
         shape, shape_times = eatube.tune_shape([eng], cluster=w["cluster"],
                                                steps=50)
         log("launch shape", shape, {k: round(v * 20, 4)

@@ -21,6 +21,23 @@ constexpr double kFnorm = 5.4210108624275218e-20;  // random_generator.h:92
 
 // ---------------------------------------------------------------- helpers
 
+// Global-memory view of the tube's arrays: the table holds generic
+// pointers, and loads through them would compile to flat loads, whose waits
+// also drain LDS traffic and serialise the samples' loads.
+typedef __attribute__((address_space(1))) uint64_t gword;
+__device__ __forceinline__ gword *gmem(uint64_t *p) { return (gword *)p; }
+__device__ __forceinline__ const gword *gmem(const uint64_t *p) { return (const gword *)p; }
+// The same for any element type.  Stores through a flat pointer also count
+// against the LDS wait counter, so an LDS read after them waits for the
+// store to reach memory; through a global view it does not.
+template <typename T>
+using global_t = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ global_t<T> *gv(T *p) {
+  return (global_t<T> *)p;
+}
+
+
 // Philox4x32-10, Salmon et al. SC'11 (the round and key schedule of
 // main.cu:996-1043).
 __device__ __forceinline__ void philox10(uint32_t &v0, uint32_t &v1,
@@ -186,8 +203,8 @@ __device__ __forceinline__ Nbrs neighbours(const Lattice &g, int p, int h) {
 // centre's copy layout; with open z boundaries the z bond of the top
 // (bottom) copy block is replaced by the y+1 (even) or y-1 (odd) neighbour
 // (main.cu:1460-1472).
-__device__ __forceinline__ void load_neighbours(const Lattice &g,
-                                                const uint64_t *old, int r,
+template <typename P>
+__device__ __forceinline__ void load_neighbours(const Lattice &g, P old, int r,
                                                 const Nbrs &nb, int parity,
                                                 uint64_t sc[6], int stride) {
 #pragma unroll
@@ -210,7 +227,7 @@ __device__ __forceinline__ void load_neighbours(const Lattice &g,
 // The six couplings of half site h of parity p of sample s.
 __device__ __forceinline__ void load_couplings(const Lattice &g, const uint64_t *J, int s, int p,
                                                int h, uint64_t out[6]) {
-  const uint64_t *j = J + coupling_index(g, s, p, h, 0);
+  const auto j = gmem(J) + coupling_index(g, s, p, h, 0);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     out[2 * k] = j[(size_t)k * 2 * g.n];
@@ -237,29 +254,25 @@ __device__ __forceinline__ void sweep_tables(const Tube &a, int beta, int step, 
                                              uint32_t *lut, C *clone_at, uint32_t (*ctr)[5]) {
   const Lattice &g = a.g;
   const int tx = threadIdx.x, T = blockDim.x;
-  const uint4 *src = reinterpret_cast<const uint4 *>(a.lut + (size_t)beta * EA_LUT_BYTES);
-  uint4 *dst = reinterpret_cast<uint4 *>(lut);
-  for (int i = tx; i < EA_LUT_BYTES / 16; i += T) dst[i] = src[i];
+  const auto src = gv(reinterpret_cast<const uint4 *>(a.lut + (size_t)beta * EA_LUT_BYTES));
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  u32x4 *dst = reinterpret_cast<u32x4 *>(lut);
+  const auto vsrc = reinterpret_cast<const global_t<u32x4> *>(src);
+  for (int i = tx; i < EA_LUT_BYTES / 16; i += T) dst[i] = vsrc[i];
   for (int i = tx; i < g.ns * g.nr; i += T) {
-    const int c = a.wctb[(size_t)i * g.nt + beta];
+    const int c = gv(a.wctb)[(size_t)i * g.nt + beta];
     clone_at[i] = ROWS ? (C)spin_row(g, i & (g.nr - 1), c) : (C)c;
   }
   if (tx < g.nr) {
-    const ea_counter c = a.ctr[((size_t)step * a.halves + half) * g.nr + tx];
-    ctr[tx][0] = c.x;
-    ctr[tx][1] = c.y;
-    ctr[tx][2] = c.z;
-    ctr[tx][3] = c.w;
+    const auto c = gv(a.ctr) + ((size_t)step * a.halves + half) * g.nr + tx;
+    ctr[tx][0] = c->x;
+    ctr[tx][1] = c->y;
+    ctr[tx][2] = c->z;
+    ctr[tx][3] = c->w;
     ctr[tx][4] = a.keys[tx];
   }
 }
 
-// Global-memory view of the tube's arrays: the table holds generic
-// pointers, and loads through them would compile to flat loads, whose waits
-// also drain LDS traffic and serialise the samples' loads.
-typedef __attribute__((address_space(1))) uint64_t gword;
-__device__ __forceinline__ gword *gmem(uint64_t *p) { return (gword *)p; }
-__device__ __forceinline__ const gword *gmem(const uint64_t *p) { return (const gword *)p; }
 
 // Sample-independent addressing of one (h, r) position of parity p: word
 // offsets of its own word and of its six neighbours within a clone row of a
@@ -432,7 +445,7 @@ __device__ __forceinline__ int group_sum(int e) {
 // random words are shared by all samples, are made once and never reach
 // HBM.  One energy partial per (sample, replica, beta, block).
 template <int POS, bool XCD>
-__global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *tubes, const uint64_t *map,
+__global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                       Phase ph) {
   static_assert(POS == 32 || POS == 64, "a sample's positions stay within one wave");
   constexpr int kSlots = kSwThreads / POS;
@@ -498,8 +511,8 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *tubes, const u
 constexpr int kLaneThreads = 256;
 
 template <bool XCD>
-__global__ void __launch_bounds__(kLaneThreads) k_sweep_lane(const Tube *tubes,
-                                                             const uint64_t *map, Phase ph) {
+__global__ void __launch_bounds__(kLaneThreads) k_sweep_lane(const Tube *__restrict__ tubes,
+                                                             const uint64_t *__restrict__ map, Phase ph) {
   __shared__ __align__(16) uint32_t lut[EA_LUT_BYTES / 4];
   __shared__ uint32_t clone_off[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
   __shared__ uint32_t ctr[EA_MAX_REPLICAS][5];
@@ -565,7 +578,7 @@ __device__ __forceinline__ int energy_blocks_x(const Lattice &g) {
   return (g.n * g.nr + 255) / 256;
 }
 
-__global__ void __launch_bounds__(256) k_energy(const Tube *tubes, const uint64_t *map,
+__global__ void __launch_bounds__(256) k_energy(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                 int step) {
   const Where wh = locate(tubes, map, step);
   const Tube &a = *wh.t;
@@ -580,7 +593,7 @@ __global__ void __launch_bounds__(256) k_energy(const Tube *tubes, const uint64_
   const int h = valid ? t >> g.nr_shift : 0, r = t & (g.nr - 1);
   int e = 0;
   if (valid) {
-    const uint64_t *base = a.spin + (size_t)s * 2 * g.half + spin_row(g, r, clone);
+    const auto base = gmem(a.spin) + (size_t)s * 2 * g.half + spin_row(g, r, clone);
     const uint64_t sv = base[h];
     const Nbrs nb = neighbours(g, 0, h);
     uint64_t sc[6], Jw[6];
@@ -594,7 +607,7 @@ __global__ void __launch_bounds__(256) k_energy(const Tube *tubes, const uint64_
   e = replica_sum(e, g.nr);
   const int lane = threadIdx.x & 63, nw = 4 * nbx;
   const int wv = bxi * 4 + (threadIdx.x >> 6);
-  if (lane < g.nr) a.part[(((size_t)s * g.nr + r) * g.nt + clone) * nw + wv] = e;
+  if (lane < g.nr) gv(a.part)[(((size_t)s * g.nr + r) * g.nt + clone) * nw + wv] = e;
 }
 
 __device__ __forceinline__ int wave_sum(int v) {
@@ -604,8 +617,8 @@ __device__ __forceinline__ int wave_sum(int v) {
 
 // Sum of the nw partials of each of a chain's nt slots into out[slot]: one
 // lane per slot when the partials are few, the whole wave per slot otherwise.
-__device__ __forceinline__ void sum_partials(const int32_t *part, int nt, int nw,
-                                             int lane, int32_t *out) {
+template <typename P>
+__device__ __forceinline__ void sum_partials(P part, int nt, int nw, int lane, int32_t *out) {
   if (nw <= 32) {
     for (int b = lane; b < nt; b += 64) {
       int acc = 0;
@@ -625,7 +638,7 @@ __device__ __forceinline__ void sum_partials(const int32_t *part, int nt, int nw
 // ener[chain][c] from the chain's partials: the sweep's partials by beta
 // (BY_BETA, clone c = wctb[chain][b]) or k_energy's partials by clone.
 template <bool BY_BETA>
-__global__ void __launch_bounds__(64) k_energy_sum(const Tube *tubes, const uint64_t *map,
+__global__ void __launch_bounds__(64) k_energy_sum(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                    int step) {
   __shared__ int32_t E[EA_MAX_BETAS];
   const Where wh = locate(tubes, map, step);
@@ -637,7 +650,7 @@ __global__ void __launch_bounds__(64) k_energy_sum(const Tube *tubes, const uint
   __syncthreads();
   for (int b = lane; b < nt; b += 64) {
     const int c = BY_BETA ? a.wctb[(size_t)chain * nt + b] : b;
-    a.ener[(size_t)chain * nt + c] = E[b];
+    gv(a.ener)[(size_t)chain * nt + c] = E[b];
   }
 }
 
@@ -663,7 +676,7 @@ __device__ __forceinline__ double exact_exp(const Tube &a, int i, int d) {
 // Both sides' errors are far below 1e-12, so every decision equals the
 // reference's.  The statistics are then added by one lane per pair with the
 // tabulated exp values.
-__global__ void __launch_bounds__(64) k_pt(const Tube *tubes, const uint64_t *map, Phase ph) {
+__global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map, Phase ph) {
   __shared__ uint64_t ira[256];
   __shared__ uint64_t zs[EA_MAX_BETAS], prs[EA_MAX_BETAS];
   __shared__ double uu[EA_MAX_BETAS], lu[EA_MAX_BETAS];
@@ -671,30 +684,54 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *tubes, const uint64_t *ma
   __shared__ uint8_t perm[EA_MAX_BETAS + 1];
   __shared__ int32_t dE[EA_MAX_BETAS];
   __shared__ uint8_t took[EA_MAX_BETAS];
+  __shared__ double ndv[EA_MAX_BETAS];
+  __shared__ double sacc[EA_MAX_BETAS], satt[EA_MAX_BETAS], sraw[EA_MAX_BETAS],
+      sesum[EA_MAX_BETAS];
+  __shared__ int32_t stlen[EA_MAX_BETAS], stoff[EA_MAX_BETAS];
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
   const int chain = wh.local, nt = g.nt, np = nt - 1, lane = threadIdx.x;
-  ea_hq_state *st = a.pt + chain;
+  // global views of the chain's arrays (see gv)
+  const auto st = gv(a.pt + chain);
+  const auto wctb = gv(a.wctb) + (size_t)chain * nt;
+  const auto ener = gv(a.ener) + (size_t)chain * nt;
+  const auto esum = gv(a.esum) + (size_t)chain * nt;
+  // everything the serial pass reads is staged before it: no global load
+  // sits on the chain of dependent swap decisions
+  const uint64_t zseed = st->zseed;
+  const uint8_t ip = st->ip;
   for (int i = lane; i < 256; i += 64) ira[i] = st->ira[i];
-  for (int i = lane; i < nt; i += 64) perm[i] = a.wctb[(size_t)chain * nt + i];
+  for (int i = lane; i < nt; i += 64) perm[i] = wctb[i];
+  const size_t k0 = (size_t)chain * np;
+  const auto acc = gv(a.acc) + k0, att = gv(a.att) + k0, raw = gv(a.raw) + k0;
+  for (int i = lane; i < np; i += 64) {
+    ndv[i] = gv(a.nd)[i];
+    stlen[i] = gv(a.tlen)[i];
+    stoff[i] = gv(a.toff)[i];
+    sacc[i] = acc[i];
+    satt[i] = att[i];
+    sraw[i] = raw[i];
+  }
+  const bool measure = ph.measure && measured_in_pt(a);
+  if (measure)
+    for (int b = lane; b < nt; b += 64) sesum[b] = esum[b];
   if (ph.energy) {
     // the sweep left partial energies by beta: dE holds them by beta for a
     // moment, E receives them by clone
-    sum_partials(a.part + (size_t)chain * nt * a.nbx, nt, a.nbx, lane, dE);
+    sum_partials(gv(a.part) + (size_t)chain * nt * a.nbx, nt, a.nbx, lane, dE);
     __syncthreads();
     for (int b = lane; b < nt; b += 64) {
       E[perm[b]] = dE[b];
-      a.ener[(size_t)chain * nt + perm[b]] = dE[b];
+      ener[perm[b]] = dE[b];
     }
     __syncthreads();
   } else {
-    for (int c = lane; c < nt; c += 64) E[c] = a.ener[(size_t)chain * nt + c];
+    for (int c = lane; c < nt; c += 64) E[c] = ener[c];
   }
-  const uint8_t ip = st->ip;
   if (lane == 0) {  // the congruential half is a cheap serial recurrence
-    uint64_t z = st->zseed;
+    uint64_t z = zseed;
     for (int k = 0; k < np; ++k) zs[k] = z = z * kLcg + 1;
   }
   __syncthreads();
@@ -731,7 +768,7 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *tubes, const uint64_t *ma
   if (lane == 0) {
     for (int i = 0; i < np; ++i) {
       const int d = E[perm[i + 1]] - E[perm[i]];
-      const double pt = a.nd[i] * (double)d;
+      const double pt = ndv[i] * (double)d;
       bool take = true;
       if (!(pt >= 0)) {
         const double u = uu[i], l = lu[i];
@@ -755,29 +792,36 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *tubes, const uint64_t *ma
   }
   __syncthreads();
   for (int i = lane; i < 256; i += 64) st->ira[i] = ira[i];
-  for (int i = lane; i < nt; i += 64) a.wctb[(size_t)chain * nt + i] = perm[i];
-  if (ph.measure && measured_in_pt(a)) {  // energy sums, history (main.cu:208-228)
+  for (int i = lane; i < nt; i += 64) wctb[i] = perm[i];
+  if (measure) {  // energy sums, history (main.cu:208-228)
     const int itb = a.it0 + wh.step;
     const int row = itb % a.frec == a.frec - 1 ? itb / a.frec : -1;
     for (int b = lane; b < nt; b += 64) {
       const int c = perm[b];
-      a.esum[(size_t)chain * nt + b] += (double)E[c];
-      if (row >= 0) a.hist[((size_t)row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
+      esum[b] = sesum[b] + (double)E[c];
+      if (row >= 0) gv(a.hist)[((size_t)row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
     }
   }
+  // statistics (upd.c:20-44): the exp of a rejected-side pair from the
+  // table of host exp values (exact_exp), with the table's bounds staged
   for (int i = lane; i < np; i += 64) {
-    const size_t k = (size_t)chain * np + i;
-    const double pt = a.nd[i] * (double)dE[i];
-    a.att[k] += 1.0;
-    a.acc[k] += pt >= 0 ? 1.0 : exact_exp(a, i, dE[i]);
-    if (took[i]) a.raw[k] += 1.0;
+    const int d = dE[i];
+    const double pt = ndv[i] * (double)d;
+    double x = 1.0;
+    if (!(pt >= 0)) {
+      const int kk = -d / 2;
+      x = kk < 1 ? exp(pt) : kk <= stlen[i] ? gv(a.tab)[stoff[i] + kk - 1] : 0.0;
+    }
+    att[i] = satt[i] + 1.0;
+    acc[i] = sacc[i] + x;
+    if (took[i]) raw[i] = sraw[i] + 1.0;
   }
 }
 
 // --------------------------------------------------------------- measures
 
 // Per-beta energy sums and the beta-of-clone history (main.cu:208-228).
-__global__ void __launch_bounds__(64) k_measure(const Tube *tubes, const uint64_t *map,
+__global__ void __launch_bounds__(64) k_measure(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                 int step) {
   const Where wh = locate(tubes, map, step);
   const Tube &a = *wh.t;
@@ -788,13 +832,13 @@ __global__ void __launch_bounds__(64) k_measure(const Tube *tubes, const uint64_
   const int row = itb % a.frec == a.frec - 1 ? itb / a.frec : -1;
   for (int b = threadIdx.x; b < nt; b += blockDim.x) {
     const int c = a.wctb[(size_t)chain * nt + b];
-    a.esum[(size_t)chain * nt + b] += (double)a.ener[(size_t)chain * nt + c];
-    if (row >= 0) a.hist[((size_t)row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
+    gv(a.esum)[(size_t)chain * nt + b] += (double)gv(a.ener)[(size_t)chain * nt + c];
+    if (row >= 0) gv(a.hist)[((size_t)row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
   }
 }
 
 // Advances the step counter of every tube that has steps left.
-__global__ void k_step_end(const Tube *tubes, int ntubes) {
+__global__ void k_step_end(const Tube *__restrict__ tubes, int ntubes) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < ntubes && *tubes[i].step < tubes[i].nsteps) *tubes[i].step += 1;
 }
@@ -861,9 +905,9 @@ struct Pair {
 };
 __device__ __forceinline__ Pair pair_words(const Lattice &g, const Tube &a, const int32_t *pairs,
                                            int s, int grp, int beta) {
-  const int r1 = pairs[2 * grp], r2 = pairs[2 * grp + 1];
-  const int c1 = a.wctb[((size_t)s * g.nr + r1) * g.nt + beta];
-  const int c2 = a.wctb[((size_t)s * g.nr + r2) * g.nt + beta];
+  const int r1 = gv(pairs)[2 * grp], r2 = gv(pairs)[2 * grp + 1];
+  const int c1 = gv(a.wctb)[((size_t)s * g.nr + r1) * g.nt + beta];
+  const int c2 = gv(a.wctb)[((size_t)s * g.nr + r2) * g.nt + beta];
   const size_t sb = (size_t)s * 2 * g.half;
   Pair q;
   q.o1 = sb + spin_row(g, r1, c1);
@@ -932,13 +976,34 @@ __device__ __forceinline__ void unite(uint32_t *lab, uint32_t a, uint32_t b) {
 // LDS, and the accesses stay LDS instructions.  Every parent index is below
 // its child's, so the forest has no cycles; a node that is not a root only
 // ever gets re-pointed at an ancestor.
+template <typename I = uint32_t>
 struct Forest {
-  uint32_t *p;
+  I *p;
   __device__ __forceinline__ uint32_t get(uint32_t i) const {
     return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __device__ __forceinline__ void put(uint32_t i, uint32_t v) const {
-    __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(p + i, (I)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  // p[a] = min(p[a], b), returning the previous p[a].  16-bit parents sit
+  // two to a 32-bit LDS word and are lowered by compare-and-swap of the
+  // word (LDS has no 16-bit atomic minimum).
+  __device__ __forceinline__ uint32_t lower(uint32_t a, uint32_t b) const {
+    if constexpr (sizeof(I) == 4) {
+      return atomicMin(p + a, b);
+    } else {
+      uint32_t *w = reinterpret_cast<uint32_t *>(p) + (a >> 1);
+      const int sh = (a & 1) * 16;
+      uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      for (;;) {
+        const uint32_t cur = (old >> sh) & 0xFFFFu;
+        if (cur <= b) return cur;
+        const uint32_t nw = (old & ~(0xFFFFu << sh)) | (b << sh);
+        if (__hip_atomic_compare_exchange_strong(w, &old, nw, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+          return cur;
+      }
+    }
   }
   __device__ __forceinline__ uint32_t root(uint32_t a) const {
     uint32_t q = get(a);
@@ -970,7 +1035,7 @@ struct Forest {
         a = b;
         b = t;
       }
-      const uint32_t old = atomicMin(p + a, b);
+      const uint32_t old = lower(a, b);
       if (old == a) return;
       a = old;
     }
@@ -1018,7 +1083,7 @@ struct Rows {
 __device__ void label_rows(const Lattice &g, uint32_t *lab, const uint64_t *rows,
                            int planes, bool wrap_z) {
   using R = Rows<uint64_t>;
-  const Forest f{lab};
+  const Forest<> f{lab};
   const int L = g.L, nrows = planes * L, T = blockDim.x, tid = threadIdx.x;
   const uint64_t full = L == 64 ? ~0ULL : ((1ULL << L) - 1);
   for (int i = tid; i < nrows * L; i += T) {
@@ -1104,8 +1169,8 @@ __device__ __forceinline__ uint64_t word_move_bits(const Lattice &g, const uint6
 // changed (the Measure_Energy that follows a cluster move, main.cu:209).
 // Dynamic LDS: rows[V/L] u64 | qw[2n] u64 | nw[2n] u64 | cb[V/32] u32 (coin
 // of every site) | forest[V] u32.
-template <typename RowT>
-__global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uint64_t *map,
+template <typename RowT, typename IdxT>
+__global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                     Phase ph) {
   using R = Rows<RowT>;
   extern __shared__ uint64_t lds[];
@@ -1113,13 +1178,14 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uin
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
+  const auto spin = gmem(a.spin);
   const Inst in = instance_of(a, wh.local, wh.step, ph.call);
   const int V = g.V, L = g.L, nrows = V / L, T = blockDim.x, tid = threadIdx.x;
   uint64_t *rows = lds;         // q rows, then move rows
   uint64_t *qw = rows + nrows;  // q words, then replica r1's new words
   uint64_t *nw = qw + 2 * g.n;  // replica r2's new words
   uint32_t *cb = reinterpret_cast<uint32_t *>(nw + 2 * g.n);
-  const Forest f{cb + V / 32};
+  const Forest<IdxT> f{reinterpret_cast<IdxT *>(cb + V / 32)};
   __shared__ int esum[2];
   const int beta = in.beta, s = in.s, grp = in.grp;
   const Pair pw = pair_words(g, a, in.pairs, s, grp, beta);
@@ -1127,7 +1193,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uin
   for (int w = tid; w < 2 * g.n; w += T) {
     const int p = w >= g.n, h = w - p * g.n;
     const size_t o = (size_t)p * g.half + h;
-    qw[w] = a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
+    qw[w] = spin[pw.o1 + o] ^ spin[pw.o2 + o];
   }
   for (int i = tid; i < V; i += T) f.p[i] = i;
   __syncthreads();
@@ -1169,7 +1235,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uin
   }
   __syncthreads();
   if (a.keep) {
-    uint32_t *out = a.keep + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
+    const auto out = gv(a.keep) + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
     for (int i = tid; i < V; i += T) {
       const int r = qdiv(i, g.mL), x = i - r * L;
       out[i] = f.root(r * L + R::rep(R::bounds((RowT)rows[r], L, full), x));
@@ -1213,9 +1279,9 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uin
     if (live && quarter == 0) {
       const size_t o = (size_t)p * g.half + h;
       const uint64_t qn = qw[w] ^ m;  // q' = q ^ coin(label)
-      const uint64_t x1 = a.spin[pw.o1 + o] ^ qn, x2 = a.spin[pw.o2 + o] ^ qn;
-      a.spin[pw.o1 + o] = x1;
-      a.spin[pw.o2 + o] = x2;
+      const uint64_t x1 = spin[pw.o1 + o] ^ qn, x2 = spin[pw.o2 + o] ^ qn;
+      spin[pw.o1 + o] = x1;
+      spin[pw.o2 + o] = x2;
       qw[w] = x1;
       nw[w] = x2;
     }
@@ -1250,14 +1316,14 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *tubes, const uin
   }
   __syncthreads();
   if (tid < 2) {
-    const int r = in.pairs[2 * grp + tid];
+    const int r = gv(in.pairs)[2 * grp + tid];
     const size_t chain = (size_t)s * g.nr + r;
-    const int c = a.wctb[chain * g.nt + beta];
-    const int old = a.ener[chain * g.nt + c];
-    a.ener[chain * g.nt + c] = esum[tid];
+    const int c = gv(a.wctb)[chain * g.nt + beta];
+    const int old = gv(a.ener)[chain * g.nt + c];
+    gv(a.ener)[chain * g.nt + c] = esum[tid];
     // the sums already hold the pre-move energy; all values are integers
     // below 2^53, so adding the difference is exact
-    if (ph.measure) a.esum[chain * g.nt + beta] += (double)(esum[tid] - old);
+    if (ph.measure) gv(a.esum)[chain * g.nt + beta] += (double)(esum[tid] - old);
   }
 }
 
@@ -1274,7 +1340,8 @@ __device__ __forceinline__ uint32_t q_site(const Lattice &g, const Tube &a,
   const int bit = bx + 4 * by + 16 * bz;
   const int h = ((x - bx * g.ML) + g.ML * (y - by * g.ML) + g.MS * (z - bz * g.MLz)) >> 1;
   const size_t o = (size_t)p * g.half + h;
-  return (uint32_t)(((a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o]) >> bit) & 1);
+  const auto spin = gmem(a.spin);
+  return (uint32_t)(((spin[pw.o1 + o] ^ spin[pw.o2 + o]) >> bit) & 1);
 }
 
 __device__ __forceinline__ size_t instance_index(const Lattice &g, const Inst &in) {
@@ -1286,7 +1353,7 @@ __device__ __forceinline__ int slab_count(const Tube &a) {
 }
 
 // Dynamic LDS: rows[planes*L] u64 | lab[planes*L*L] u32.
-__global__ void __launch_bounds__(1024) k_hou_slab(const Tube *tubes, const uint64_t *map,
+__global__ void __launch_bounds__(1024) k_hou_slab(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                    Phase ph) {
   extern __shared__ uint64_t lds[];
   const Where wh = locate(tubes, map, ph.step);
@@ -1306,20 +1373,20 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *tubes, const uint
   for (int r = tid; r < planes * g.L; r += T)
     rows[r] = row_bits(g, z0 * g.L + r, [&](int p, int h) {
       const size_t o = (size_t)p * g.half + h;
-      return a.spin[pw.o1 + o] ^ a.spin[pw.o2 + o];
+      return gmem(a.spin)[pw.o1 + o] ^ gmem(a.spin)[pw.o2 + o];
     });
   const uint64_t cidx = in.coin + (uint64_t)in.s * g.V * g.nt + (uint64_t)in.beta * g.V + gbase;
   for (int c = tid; c < nsite / 32; c += T)
-    a.hcb[ib * (g.V / 32) + gbase / 32 + c] = coin_word(a.coin_seed, cidx + 32 * c);
+    gv(a.hcb)[ib * (g.V / 32) + gbase / 32 + c] = coin_word(a.coin_seed, cidx + 32 * c);
   __syncthreads();
   label_rows(g, lab, rows, planes, false);
-  uint32_t *gl = a.hlab + ib * g.V + gbase;
+  const auto gl = gv(a.hlab) + ib * g.V + gbase;
   for (int i = tid; i < nsite; i += T) gl[i] = gbase + lab[i];
 }
 
 // z bonds between plane z0-1 (or Lz-1 for the periodic wrap) and plane z0,
 // the first plane of the slab.
-__global__ void __launch_bounds__(256) k_hou_seam(const Tube *tubes, const uint64_t *map,
+__global__ void __launch_bounds__(256) k_hou_seam(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                   Phase ph) {
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
@@ -1334,7 +1401,7 @@ __global__ void __launch_bounds__(256) k_hou_seam(const Tube *tubes, const uint6
     zl = g.Lz - 1;
   }
   const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
-  uint32_t *gl = a.hlab + instance_index(g, in) * g.V;
+  uint32_t *gl = a.hlab + instance_index(g, in) * g.V;  // global unions, no LDS
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     const int u = z0 * S + i, d = zl * S + i;
     if (q_site(g, a, pw, u) == q_site(g, a, pw, d)) unite(gl, u, d);
@@ -1344,7 +1411,7 @@ __global__ void __launch_bounds__(256) k_hou_seam(const Tube *tubes, const uint6
 __device__ __forceinline__ int site_blocks(const Lattice &g) { return (g.V + 255) / 256; }
 __device__ __forceinline__ int word_blocks(const Lattice &g) { return (2 * g.n + 255) / 256; }
 
-__global__ void __launch_bounds__(256) k_hou_flatten(const Tube *tubes, const uint64_t *map,
+__global__ void __launch_bounds__(256) k_hou_flatten(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                      Phase ph) {
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
@@ -1353,15 +1420,15 @@ __global__ void __launch_bounds__(256) k_hou_flatten(const Tube *tubes, const ui
   const int nb = site_blocks(g), bxi = wh.local % nb;
   const Inst in = instance_of(a, wh.local / nb, wh.step, ph.call);
   const size_t ib = instance_index(g, in);
-  uint32_t *gl = a.hlab + ib * g.V;
+  uint32_t *gl = a.hlab + ib * g.V;  // global finds, no LDS
   const int i = bxi * blockDim.x + threadIdx.x;
   if (i >= g.V) return;
   const uint32_t root = find_root(gl, i);
   gl[i] = root;
-  if (a.keep) a.keep[ib * g.V + i] = root;
+  if (a.keep) gv(a.keep)[ib * g.V + i] = root;
 }
 
-__global__ void __launch_bounds__(256) k_hou_apply(const Tube *tubes, const uint64_t *map,
+__global__ void __launch_bounds__(256) k_hou_apply(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                    Phase ph) {
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
@@ -1372,8 +1439,8 @@ __global__ void __launch_bounds__(256) k_hou_apply(const Tube *tubes, const uint
   const int w = bxi * blockDim.x + threadIdx.x;
   if (w >= 2 * g.n) return;
   const size_t ib = instance_index(g, in);
-  const uint32_t *gl = a.hlab + ib * g.V;
-  const uint32_t *cb = a.hcb + ib * (g.V / 32);
+  const auto gl = gv(a.hlab) + ib * g.V;
+  const auto cb = gv(a.hcb) + ib * (g.V / 32);
   const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
   const int p = w >= g.n, h = w - p * g.n;
   const size_t o = (size_t)p * g.half + h;
@@ -1383,10 +1450,11 @@ __global__ void __launch_bounds__(256) k_hou_apply(const Tube *tubes, const uint
     const uint32_t root = gl[base + copy_offset(g, b)];
     m |= (uint64_t)((cb[root >> 5] >> (root & 31)) & 1u) << b;
   }
-  const uint64_t x1 = a.spin[pw.o1 + o], x2 = a.spin[pw.o2 + o];
+  const auto spin = gmem(a.spin);
+  const uint64_t x1 = spin[pw.o1 + o], x2 = spin[pw.o2 + o];
   const uint64_t qn = x1 ^ x2 ^ m;
-  a.spin[pw.o1 + o] = x1 ^ qn;
-  a.spin[pw.o2 + o] = x2 ^ qn;
+  spin[pw.o1 + o] = x1 ^ qn;
+  spin[pw.o2 + o] = x2 ^ qn;
 }
 
 inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -1413,9 +1481,12 @@ size_t energy_partials(const Lattice &g) {
                                 4 * (size_t)energy_blocks_host(g));
 }
 
+// 16-bit parents when every site index fits (and the rows are 32 bits).
+static bool hou_idx16(const Lattice &g) { return g.L <= 32 && g.V <= 65536; }
+
 size_t hou_whole_lds(const Lattice &g) {
   return 8 * (size_t)(g.V / g.L) + 32 * (size_t)g.n + 4 * (size_t)(g.V / 32) +
-         4 * (size_t)g.V;
+         (hou_idx16(g) ? 2 : 4) * (size_t)g.V;
 }
 
 size_t hou_slab_lds(const Lattice &g, int planes) {
@@ -1443,7 +1514,10 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
     }
     const long long inst = 2LL * g.ns * g.ctm;
     if (t.planes == 0) {
-      if (g.L <= 32) {
+      if (hou_idx16(g)) {
+        add(kMapWhole16, inst);
+        p.lds16 = std::max(p.lds16, hou_whole_lds(g));
+      } else if (g.L <= 32) {
         add(kMapWhole32, inst);
         p.lds32 = std::max(p.lds32, hou_whole_lds(g));
       } else {
@@ -1470,6 +1544,7 @@ hipError_t upload_plan(const Plan &p, DevPlan *d) {
     d->cnt[k] = p.map[k].size();
     total += p.map[k].size();
   }
+  d->lds16 = p.lds16;
   d->lds32 = p.lds32;
   d->lds64 = p.lds64;
   d->lds_slab = p.lds_slab;
@@ -1582,15 +1657,24 @@ hipError_t launch_step_end(const Tube *tubes, int ntubes, hipStream_t s) {
 
 
 hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s) {
+  const dim3 wg(p.shape.cluster_threads);
+  if (p.cnt[kMapWhole16]) {
+    auto k = k_hou_whole<uint32_t, uint16_t>;
+    allow_lds(reinterpret_cast<const void *>(k), p.lds16);
+    hipLaunchKernelGGL(k, dim3((unsigned)p.cnt[kMapWhole16]), wg, p.lds16, s, tubes,
+                       p.map(kMapWhole16), ph);
+  }
   if (p.cnt[kMapWhole32]) {
-    allow_lds(reinterpret_cast<const void *>(k_hou_whole<uint32_t>), p.lds32);
-    hipLaunchKernelGGL(k_hou_whole<uint32_t>, dim3((unsigned)p.cnt[kMapWhole32]),
-                       dim3(p.shape.cluster_threads), p.lds32, s, tubes, p.map(kMapWhole32), ph);
+    auto k = k_hou_whole<uint32_t, uint32_t>;
+    allow_lds(reinterpret_cast<const void *>(k), p.lds32);
+    hipLaunchKernelGGL(k, dim3((unsigned)p.cnt[kMapWhole32]), wg, p.lds32, s, tubes,
+                       p.map(kMapWhole32), ph);
   }
   if (p.cnt[kMapWhole64]) {
-    allow_lds(reinterpret_cast<const void *>(k_hou_whole<uint64_t>), p.lds64);
-    hipLaunchKernelGGL(k_hou_whole<uint64_t>, dim3((unsigned)p.cnt[kMapWhole64]),
-                       dim3(p.shape.cluster_threads), p.lds64, s, tubes, p.map(kMapWhole64), ph);
+    auto k = k_hou_whole<uint64_t, uint32_t>;
+    allow_lds(reinterpret_cast<const void *>(k), p.lds64);
+    hipLaunchKernelGGL(k, dim3((unsigned)p.cnt[kMapWhole64]), wg, p.lds64, s, tubes,
+                       p.map(kMapWhole64), ph);
   }
   if (p.cnt[kMapSlab]) {
     allow_lds(reinterpret_cast<const void *>(k_hou_slab), p.lds_slab);

@@ -104,7 +104,7 @@ struct Phase {
 // kMapEnergyU / kMapChainU cover only the tubes whose step measures with a
 // separate energy pass (cluster moves on the slab path).
 enum MapKind {
-  kMapSweep, kMapEnergy, kMapChain, kMapWhole32, kMapWhole64, kMapSlab,
+  kMapSweep, kMapEnergy, kMapChain, kMapWhole16, kMapWhole32, kMapWhole64, kMapSlab,
   kMapSeam, kMapFlatten, kMapApply, kMapEnergyU, kMapChainU, kMapKinds
 };
 
@@ -135,7 +135,7 @@ constexpr int kSweepPosMin = 32;
 struct Plan {
   std::vector<uint64_t> map[kMapKinds];
   Shape shape;
-  size_t lds32 = 0, lds64 = 0, lds_slab = 0;
+  size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0;
   int ntubes = 0;
 };
 
@@ -146,7 +146,7 @@ struct DevPlan {
   uint64_t *buf = nullptr;
   size_t off[kMapKinds] = {};
   size_t cnt[kMapKinds] = {};
-  size_t lds32 = 0, lds64 = 0, lds_slab = 0;
+  size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0;
   Shape shape;
   bool identity = false;
   const uint64_t *map(int k) const { return identity ? nullptr : buf + off[k]; }
