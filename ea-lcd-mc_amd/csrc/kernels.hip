@@ -1080,10 +1080,11 @@ struct Rows {
 //     two runs the union is implied);
 //  3. flatten.
 // z bonds leave the slab only when wrap_z (the whole periodic tube).
-__device__ void label_rows(const Lattice &g, uint32_t *lab, const uint64_t *rows,
-                           int planes, bool wrap_z) {
+template <typename I>
+__device__ void label_rows(const Lattice &g, I *lab, const uint64_t *rows, int planes,
+                           bool wrap_z) {
   using R = Rows<uint64_t>;
-  const Forest<> f{lab};
+  const Forest<I> f{lab};
   const int L = g.L, nrows = planes * L, T = blockDim.x, tid = threadIdx.x;
   const uint64_t full = L == 64 ? ~0ULL : ((1ULL << L) - 1);
   for (int i = tid; i < nrows * L; i += T) {
@@ -1328,22 +1329,10 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
 }
 
 // Tiled Houdayer for tubes larger than one workgroup's LDS.  Pass 1 labels
-// each slab of `planes` z-planes in LDS and stores global labels; pass 2
-// unites across slab boundaries in global memory; pass 3 flattens; pass 4
-// applies the move.  Slab block local index: slab + nslabs * instance.
-__device__ __forceinline__ uint32_t q_site(const Lattice &g, const Tube &a,
-                                           const Pair &pw, int site) {
-  const int yz = qdiv(site, g.mL), x = site - yz * g.L;
-  const int z = qdiv(yz, g.mL), y = yz - z * g.L;
-  const int p = (x ^ y ^ z) & 1;
-  const int bx = qdiv(x, g.mML), by = qdiv(y, g.mML), bz = qdiv(z, g.mMLz);
-  const int bit = bx + 4 * by + 16 * bz;
-  const int h = ((x - bx * g.ML) + g.ML * (y - by * g.ML) + g.MS * (z - bz * g.MLz)) >> 1;
-  const size_t o = (size_t)p * g.half + h;
-  const auto spin = gmem(a.spin);
-  return (uint32_t)(((spin[pw.o1 + o] ^ spin[pw.o2 + o]) >> bit) & 1);
-}
-
+// each slab of `planes` z-planes in LDS and stores global labels (each
+// site's slab-local root); pass 2 unites across slab boundaries in global
+// memory; pass 3 finds every site's root and applies the move.  Slab block
+// local index: slab + nslabs * instance.
 __device__ __forceinline__ size_t instance_index(const Lattice &g, const Inst &in) {
   return ((size_t)in.grp * g.ns + in.s) * g.ctm + in.beta;
 }
@@ -1366,7 +1355,7 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *__restrict__ tube
   const int planes = min(a.planes, g.Lz - z0), nsite = planes * S;
   const int T = blockDim.x, tid = threadIdx.x;
   uint64_t *rows = lds;
-  uint32_t *lab = reinterpret_cast<uint32_t *>(rows + a.planes * g.L);
+  uint16_t *lab = reinterpret_cast<uint16_t *>(rows + a.planes * g.L);  // < 65536 sites
   const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
   const size_t ib = instance_index(g, in);
   const int gbase = z0 * S;
@@ -1385,16 +1374,21 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *__restrict__ tube
 }
 
 // z bonds between plane z0-1 (or Lz-1 for the periodic wrap) and plane z0,
-// the first plane of the slab.
-__global__ void __launch_bounds__(256) k_hou_seam(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
-                                                  Phase ph) {
+// the first plane of the slab, one lane per row y: the two rows' q bits
+// (row_bits) give the overlaps of equal-q runs, and each overlap is united
+// once, at its start, between the two runs' representatives (as in the
+// whole-tube labelling).  The sites of a run already share a root.
+__global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
+                                                 const uint64_t *__restrict__ map, Phase ph) {
+  using R = Rows<uint64_t>;
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
   const int ns = slab_count(a), slab = wh.local % ns;
   const Inst in = instance_of(a, wh.local / ns, wh.step, ph.call);
-  const int S = g.L * g.L, z0 = slab * a.planes;
+  const int L = g.L, y = threadIdx.x, z0 = slab * a.planes;
+  if (y >= L) return;  // no barrier follows
   int zl = z0 - 1;
   if (z0 == 0) {
     if (g.bc != EA_PBC) return;
@@ -1402,34 +1396,33 @@ __global__ void __launch_bounds__(256) k_hou_seam(const Tube *__restrict__ tubes
   }
   const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
   uint32_t *gl = a.hlab + instance_index(g, in) * g.V;  // global unions, no LDS
-  for (int i = threadIdx.x; i < S; i += blockDim.x) {
-    const int u = z0 * S + i, d = zl * S + i;
-    if (q_site(g, a, pw, u) == q_site(g, a, pw, d)) unite(gl, u, d);
+  const auto spin = gmem(a.spin);
+  auto qword = [&](int p, int h) {
+    const size_t o = (size_t)p * g.half + h;
+    return spin[pw.o1 + o] ^ spin[pw.o2 + o];
+  };
+  const uint64_t full = L == 64 ? ~0ULL : ((1ULL << L) - 1);
+  const uint64_t qu = row_bits(g, z0 * L + y, qword), qd = row_bits(g, zl * L + y, qword);
+  const uint64_t bu = R::bounds(qu, L, full), bd = R::bounds(qd, L, full);
+  const uint64_t lu = ~(qu ^ (qu << 1)) & full & ~1ULL;  // q[x] == q[x-1]
+  const uint64_t ld = ~(qd ^ (qd << 1)) & full & ~1ULL;
+  uint64_t todo = ~(qu ^ qd) & full & ~(lu & ld);
+  const uint32_t ru = (uint32_t)(z0 * L + y) * L, rd = (uint32_t)(zl * L + y) * L;
+  while (todo) {
+    const int x = R::lo(todo);
+    todo &= todo - 1;
+    unite(gl, ru + R::rep(bu, x), rd + R::rep(bd, x));
   }
 }
 
-__device__ __forceinline__ int site_blocks(const Lattice &g) { return (g.V + 255) / 256; }
 __device__ __forceinline__ int word_blocks(const Lattice &g) { return (2 * g.n + 255) / 256; }
 
-__global__ void __launch_bounds__(256) k_hou_flatten(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
-                                                     Phase ph) {
-  const Where wh = locate(tubes, map, ph.step);
-  const Tube &a = *wh.t;
-  if (wh.step >= a.nsteps) return;
-  const Lattice g = a.g;
-  const int nb = site_blocks(g), bxi = wh.local % nb;
-  const Inst in = instance_of(a, wh.local / nb, wh.step, ph.call);
-  const size_t ib = instance_index(g, in);
-  uint32_t *gl = a.hlab + ib * g.V;  // global finds, no LDS
-  const int i = bxi * blockDim.x + threadIdx.x;
-  if (i >= g.V) return;
-  const uint32_t root = find_root(gl, i);
-  gl[i] = root;
-  if (a.keep) gv(a.keep)[ib * g.V + i] = root;
-}
-
-__global__ void __launch_bounds__(256) k_hou_apply(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
-                                                   Phase ph) {
+// The move: each of a word's 64 sites finds its cluster root (a slab's
+// label is the slab-local root; the seams linked those roots, so the chain
+// is short), and the root's coin decides.  With labels kept, the roots are
+// stored too.
+__global__ void __launch_bounds__(256) k_hou_apply(const Tube *__restrict__ tubes,
+                                                   const uint64_t *__restrict__ map, Phase ph) {
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
@@ -1439,7 +1432,7 @@ __global__ void __launch_bounds__(256) k_hou_apply(const Tube *__restrict__ tube
   const int w = bxi * blockDim.x + threadIdx.x;
   if (w >= 2 * g.n) return;
   const size_t ib = instance_index(g, in);
-  const auto gl = gv(a.hlab) + ib * g.V;
+  const uint32_t *gl = a.hlab + ib * g.V;
   const auto cb = gv(a.hcb) + ib * (g.V / 32);
   const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
   const int p = w >= g.n, h = w - p * g.n;
@@ -1447,8 +1440,10 @@ __global__ void __launch_bounds__(256) k_hou_apply(const Tube *__restrict__ tube
   const int base = site_base(g, p, h);
   uint64_t m = 0;
   for (int b = 0; b < 64; ++b) {
-    const uint32_t root = gl[base + copy_offset(g, b)];
+    const int site = base + copy_offset(g, b);
+    const uint32_t root = find_root(gl, site);
     m |= (uint64_t)((cb[root >> 5] >> (root & 31)) & 1u) << b;
+    if (a.keep) gv(a.keep)[ib * g.V + site] = root;
   }
   const auto spin = gmem(a.spin);
   const uint64_t x1 = spin[pw.o1 + o], x2 = spin[pw.o2 + o];
@@ -1490,7 +1485,7 @@ size_t hou_whole_lds(const Lattice &g) {
 }
 
 size_t hou_slab_lds(const Lattice &g, int planes) {
-  return 8 * (size_t)planes * g.L + 4 * (size_t)planes * g.L * g.L;
+  return 8 * (size_t)planes * g.L + 2 * (size_t)planes * g.L * g.L;
 }
 
 Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
@@ -1528,7 +1523,6 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
       const long long slabs = (g.Lz + t.planes - 1) / t.planes;
       add(kMapSlab, slabs * inst);
       add(kMapSeam, slabs * inst);
-      add(kMapFlatten, (long long)blocks_for(g.V, 256) * inst);
       add(kMapApply, (long long)blocks_for(2LL * g.n, 256) * inst);
       p.lds_slab = std::max(p.lds_slab, hou_slab_lds(g, t.planes));
     }
@@ -1680,10 +1674,8 @@ hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStr
     allow_lds(reinterpret_cast<const void *>(k_hou_slab), p.lds_slab);
     hipLaunchKernelGGL(k_hou_slab, dim3((unsigned)p.cnt[kMapSlab]), dim3(1024), p.lds_slab, s,
                        tubes, p.map(kMapSlab), ph);
-    hipLaunchKernelGGL(k_hou_seam, dim3((unsigned)p.cnt[kMapSeam]), dim3(256), 0, s, tubes,
+    hipLaunchKernelGGL(k_hou_seam, dim3((unsigned)p.cnt[kMapSeam]), dim3(64), 0, s, tubes,
                        p.map(kMapSeam), ph);
-    hipLaunchKernelGGL(k_hou_flatten, dim3((unsigned)p.cnt[kMapFlatten]), dim3(256), 0, s,
-                       tubes, p.map(kMapFlatten), ph);
     hipLaunchKernelGGL(k_hou_apply, dim3((unsigned)p.cnt[kMapApply]), dim3(256), 0, s, tubes,
                        p.map(kMapApply), ph);
   }

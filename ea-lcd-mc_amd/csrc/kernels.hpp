@@ -105,7 +105,7 @@ struct Phase {
 // separate energy pass (cluster moves on the slab path).
 enum MapKind {
   kMapSweep, kMapEnergy, kMapChain, kMapWhole16, kMapWhole32, kMapWhole64, kMapSlab,
-  kMapSeam, kMapFlatten, kMapApply, kMapEnergyU, kMapChainU, kMapKinds
+  kMapSeam, kMapApply, kMapEnergyU, kMapChainU, kMapKinds
 };
 
 // A tube's step measures its energies inside PT (and the whole-tube cluster
@@ -184,7 +184,7 @@ hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
 // otherwise in slabs of about kHouSlabSites sites.  Both need L <= 64.
 size_t hou_whole_lds(const Lattice &g);
 size_t hou_slab_lds(const Lattice &g, int planes);
-constexpr int kHouSlabSites = 16384;
+constexpr int kHouSlabSites = 65536;  // slab-local labels are 16-bit
 constexpr int kHouMaxL = 64;
 
 }  // namespace eatube

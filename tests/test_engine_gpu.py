@@ -124,8 +124,9 @@ def test_steps_with_houdayer_match_oracle(case):
 
 @pytest.mark.parametrize("bc", [0, 1])
 def test_tiled_houdayer_matches_oracle(bc):
-    """V = 16*16*136 exceeds one workgroup's LDS: slab labelling + merge."""
-    o, e = pair(16, 136, 4, 4, 2, bc=bc, seeds=(21, 22, 23, 24))
+    """V = 16*16*264 = 67584 sites: beyond 16-bit whole-tube labels, and
+    32-bit labels exceed one workgroup's LDS: slab labelling + merge."""
+    o, e = pair(16, 264, 4, 4, 2, bc=bc, seeds=(21, 22, 23, 24))
     o.keep_labels()
     e.engine.keep_labels()
     e.steps(2, cluster=True)
