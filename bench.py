@@ -217,24 +217,36 @@ def serving(nreq, rate, sweeps, budget_gb=16.0, chunk=32):
 
 
 def cpu_baseline(w, S, budget_s):
-    """The oracle (scalar C restatement of the reference algorithm) on one
-    host core, over a bounded number of steps of the same workload."""
+    """The oracle (scalar C restatement of the reference algorithm) on the
+    host: all the job's cores (OpenMP over samples, chains and cluster
+    instances, OMP_NUM_THREADS or every core) and one core, each over a
+    bounded number of steps of the same workload (half the budget each)."""
     import orc
     betas = eatube.betas_linear(w["tmin"], w["tmax"], w["nt"])
-    o = orc.Sim(w["L"], w["Lz"], w["nr"], w["nt"], S, bc=w["bc"],
-                betas=betas, seeds=SEEDS)
-    t0 = time.perf_counter()
-    o.steps(1, cluster=w["cluster"])
-    t1 = time.perf_counter() - t0
-    n = int(max(1, min(200, budget_s / max(t1, 1e-6))))
-    t0 = time.perf_counter()
-    o.steps(n, cluster=w["cluster"])
-    dt = time.perf_counter() - t0
     V = w["L"] * w["L"] * w["Lz"]
-    return {"value": n * V * w["nr"] * w["nt"] * S / (dt * 1e9),
-            "unit": "spin-flip attempts/ns", "cores": 1, "kind": "port",
-            "sample": "%d steps of the same workload (all %d samples), "
-                      "%.1f s on one core" % (n, S, dt)}
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+
+    def rate(threads, seconds):
+        orc.set_threads(threads)
+        o = orc.Sim(w["L"], w["Lz"], w["nr"], w["nt"], S, bc=w["bc"],
+                    betas=betas, seeds=SEEDS)
+        t0 = time.perf_counter()
+        o.steps(1, cluster=w["cluster"])
+        t1 = time.perf_counter() - t0
+        n = int(max(1, min(2000, seconds / max(t1, 1e-6))))
+        t0 = time.perf_counter()
+        o.steps(n, cluster=w["cluster"])
+        dt = time.perf_counter() - t0
+        return n * V * w["nr"] * w["nt"] * S / (dt * 1e9), n, dt
+
+    v_all, n_all, dt_all = rate(cores, budget_s / 2)
+    v_one, n_one, dt_one = rate(1, budget_s / 2)
+    orc.set_threads(1)
+    return {"value": v_all, "unit": "spin-flip attempts/ns", "cores": cores,
+            "kind": "port", "one_core_value": v_one,
+            "sample": "%d steps of the same workload (all %d samples) in "
+                      "%.1f s on %d threads; %d steps in %.1f s on one core"
+                      % (n_all, S, dt_all, cores, n_one, dt_one)}
 
 
 def main():
@@ -245,7 +257,7 @@ def main():
     ap.add_argument("--workload", default="obc_L16_Lz64",
                     choices=sorted(WORKLOADS))
     ap.add_argument("--samples", type=int, default=31)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-tune", action="store_true",

@@ -10,6 +10,7 @@
 #include "eaoracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <quadmath.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -1001,16 +1002,22 @@ void orc_sim_free(orc_sim *s) {
 }
 
 /* Monte_Carlo, main.cu:285-349 */
+/* Threads the simulation loops use (samples, chains and cluster instances
+ * are independent, so any count gives the same results); 1 by default. */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
 void orc_sim_sweeps(orc_sim *s, int nsweeps) {
   const int nr = s->c.nr, nt = s->c.nt, n = s->n;
   const size_t half = (size_t)n * nr * nt;
   uint32_t *ctr = malloc(sizeof(uint32_t) * 4 * nr);
-  uint32_t *wc = malloc(sizeof(uint32_t) * nt * nr);
   for (int sw = 0; sw < nsweeps; sw++)
     for (int par = 0; par < 2; par++) {
       orc_fill_bits(s->xo, s->prc, nr, s->iter, par, s->c.job_id, ctr);
       orc_rand_words(n, nr, nt, ctr, s->keys, s->lut, s->rw);
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic)
       for (int i = 0; i < s->c.ns; i++) {
+        uint32_t wc[256 * 16];
         /* d_whichclone[c*NR + r] = which_beta_this_clon (main.cu:733-738) */
         for (int r = 0; r < nr; r++)
           for (int c = 0; c < nt; c++)
@@ -1021,13 +1028,13 @@ void orc_sim_sweeps(orc_sim *s, int nsweeps) {
       s->iter++;
     }
   free(ctr);
-  free(wc);
 }
 
 /* Measure_Energy, main.cu:531-554 */
 void orc_sim_energy(orc_sim *s) {
   const int nr = s->c.nr, nt = s->c.nt, n = s->n;
   const size_t half = (size_t)n * nr * nt;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic)
   for (int i = 0; i < s->c.ns; i++)
     orc_energy_msc(&s->t, nr, nt, s->c.bc, s->spin + 2 * half * i,
                    s->J + (size_t)12 * n * i, s->ener + (size_t)i * nr * nt);
@@ -1036,6 +1043,7 @@ void orc_sim_energy(orc_sim *s) {
 /* PT loop of main.cu:170-178 */
 void orc_sim_pt(orc_sim *s) {
   const int nr = s->c.nr, nt = s->c.nt;
+#pragma omp parallel for num_threads(g_threads) collapse(2) schedule(dynamic)
   for (int i = 0; i < s->c.ns; i++)
     for (int r = 0; r < nr; r++) {
       size_t k = (size_t)i * nr + r;
@@ -1050,8 +1058,6 @@ void orc_sim_houdayer(orc_sim *s) {
   const int nr = s->c.nr, nt = s->c.nt, ns = s->c.ns, n = s->n, V = s->V;
   const int L = s->c.L, Lz = s->c.Lz, ctm = s->c.clonetbm;
   const size_t half = (size_t)n * nr * nt;
-  uint8_t *q = malloc((size_t)V);
-  uint32_t *lab = malloc(sizeof(uint32_t) * V);
   if (s->keep_labels && !s->last_labels)
     s->last_labels = malloc(sizeof(uint32_t) * 2 * ns * ctm * (size_t)V);
   for (int grp = 0; grp < 2; grp++) {
@@ -1068,9 +1074,13 @@ void orc_sim_houdayer(orc_sim *s) {
     const int r1 = index[0], r2 = index[1];
     s->last_pairs[2 * grp] = r1;
     s->last_pairs[2 * grp + 1] = r2;
+    /* the (sample, beta) instances of a group touch disjoint words */
+#pragma omp parallel for num_threads(g_threads) collapse(2) schedule(dynamic)
     for (int i = 0; i < ns; i++) {
-      uint64_t *sp = s->spin + 2 * half * i;
       for (int b = 0; b < ctm; b++) {
+        uint64_t *sp = s->spin + 2 * half * i;
+        uint8_t *q = malloc((size_t)V);
+        uint32_t *lab = malloc(sizeof(uint32_t) * V);
         int c1 = s->wctb[((size_t)i * nr + r1) * nt + b];
         int c2 = s->wctb[((size_t)i * nr + r2) * nt + b];
         /* d_computeoverlap + d_unpack */
@@ -1096,12 +1106,12 @@ void orc_sim_houdayer(orc_sim *s) {
           sp[par * half + (size_t)c1 * n * nr + (size_t)h * nr + r1] ^= m;
           sp[par * half + (size_t)c2 * n * nr + (size_t)h * nr + r2] ^= m;
         }
+        free(q);
+        free(lab);
       }
     }
     s->rng_offset += (uint64_t)V * nt * ns; /* cuda_rng_offset, :195 */
   }
-  free(q);
-  free(lab);
 }
 
 void orc_sim_steps(orc_sim *s, int nsteps, int ptfr, int houfr, int cluster) {

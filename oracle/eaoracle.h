@@ -209,6 +209,8 @@ void orc_sim_houdayer(orc_sim *s);
 /* nsteps of: houfr x (ptfr sweeps, energy, PT), then Houdayer (if on),
  * then energy -- the body of main.cu:149-218 for one `it` with mesfr=1 */
 void orc_sim_steps(orc_sim *s, int nsteps, int ptfr, int houfr, int cluster);
+/* Threads of the simulation loops (default 1); results do not depend on it. */
+void orc_set_threads(int n);
 orc_sim *orc_sim_create(const orc_cfg *c);
 void orc_sim_destroy(orc_sim *s);
 void orc_sim_keep_labels(orc_sim *s, int on);

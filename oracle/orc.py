@@ -108,6 +108,7 @@ def lib():
             "orc_sim_pt": (None, [vp]),
             "orc_sim_houdayer": (None, [vp]),
             "orc_sim_steps": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "orc_set_threads": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -227,6 +228,12 @@ def betas_linear(tmin, tmax, nt):
     """betas = 1/T, T linearly spaced in [tmin, tmax], sorted descending."""
     t = np.linspace(tmin, tmax, nt)
     return np.array(sorted((1.0 / t).tolist(), reverse=True))
+
+
+def set_threads(n):
+    """Threads of the oracle's sample/chain/instance loops (results do not
+    depend on it); 1 unless set."""
+    lib().orc_set_threads(int(n))
 
 
 class Sim:
