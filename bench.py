@@ -200,15 +200,20 @@ def serving(nreq, rate, sweeps, budget_gb=16.0, chunk=32):
     whole-run throughput."""
     import serve
     reqs = serve.poisson_requests(nreq, rate, seed=5, sweeps=sweeps)
-    srv = serve.Server(budget=int(budget_gb * 2 ** 30), chunk=chunk)
+    workers = min(8, int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1)
+    srv = serve.Server(budget=int(budget_gb * 2 ** 30), chunk=chunk, workers=workers)
     t0 = time.perf_counter()
     res = srv.run(reqs)
     wall = time.perf_counter() - t0
     out = serve.summary(reqs, res, wall)
     out.update({"rate_per_s": rate, "sweeps_range": list(sweeps),
                 "chunk_steps": chunk, "hbm_budget_gb": budget_gb,
+                "prepare_workers": workers,
                 "max_running": srv.stats["max_running"],
                 "batches_built": srv.stats["batches_built"],
+                "chunks": srv.stats["chunks"],
+                "host_seconds": {k[2:]: round(v, 3) for k, v in srv.stats.items()
+                                 if k.startswith("t_")},
                 "request_shape": "L in {8,16,24,32}, Lz multiple of 8 in "
                                  "[L,16L], PBC|OBC, NT in {16,24,32} in "
                                  "[0.6,1.6], 4 replicas, 1 sample, sweep+PT "

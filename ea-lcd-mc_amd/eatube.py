@@ -414,27 +414,24 @@ class Batch:
 
 # ------------------------------------------------------------- simulation
 
-class Simulation:
-    """Seeds -> generators -> samples -> engine, as main() and Init() do
-    (main.cu:70-119, ini.c:8-101), then the step loop on the device."""
+class Prepared:
+    """The host half of a Simulation: seeds, generators, PT states and the
+    packed samples (ini.c:8-101), made without touching the GPU, so it can
+    run on a worker thread (the library calls release the interpreter).
 
-    def __init__(self, L, Lz, nr, nt, ns, bc=PBC, betas=None, lut=None,
-                 seeds=(1, 2, 3, 4), job_id=0, variant=0, mattis=0,
-                 list_samples=None, clone_thr=None):
-        if betas is None:
-            raise ValueError("betas required")
-        self.L, self.Lz, self.nr, self.nt, self.ns = L, Lz, nr, nt, ns
-        self.bc, self.job = bc, job_id
-        self.V = L * L * Lz
-        self.seed_J, self.seed_u, self.seed_MC, self.seed_cl = seeds
+    generate_seeds_from_one emits its seeds in sequence, so only as many as
+    the sample list reaches are made (the reference makes 32 and picks)."""
+
+    def __init__(self, L, Lz, nr, nt, ns, bc=PBC, seeds=(1, 2, 3, 4),
+                 variant=0, mattis=0, list_samples=None):
+        seed_J, seed_u, seed_MC, seed_cl = seeds
         ls = list(range(ns)) if list_samples is None else list(list_samples)
         self.list_samples = ls
-        tJ = seeds_from_one(self.seed_J, 32)
-        tu = seeds_from_one(self.seed_u, 32)
-        self.seeds_J = tJ[ls]
-        self.seeds_u = tu[ls]
-        self.seeds_MC = seeds_from_one(self.seed_MC, nr)
-        self.cl = hq_init(self.seed_cl)
+        upto = max(ls) + 1
+        self.seeds_J = seeds_from_one(seed_J, upto)[ls]
+        self.seeds_u = seeds_from_one(seed_u, upto)[ls]
+        self.seeds_MC = seeds_from_one(seed_MC, nr)
+        self.cl = hq_init(seed_cl)
         self.prc = (HQ * nr)()
         self.xo = (XO * nr)()
         self.keys = np.zeros(nr, np.uint32)
@@ -443,27 +440,49 @@ class Simulation:
             lib().ea_init_generators(int(self.seeds_MC[r]), C.byref(self.prc[r]),
                                      C.byref(self.xo[r]), C.byref(k))
             self.keys[r] = k.value
-        pts = (HQ * (ns * nr))()
+        self.pts = (HQ * (ns * nr))()
         for r in range(nr):
-            t = seeds_from_one(int(self.seeds_MC[r]), 32)
+            t = seeds_from_one(int(self.seeds_MC[r]), upto)
             for i, smp in enumerate(ls):
                 s32 = int(t[smp]) & 0xFFFFFFFF  # `int seed_PT`, ini.c:63,90
                 if s32 >= 2**31:
                     s32 -= 2**32
-                lib().ea_hq_init(C.byref(pts[i * nr + r]), s32 & (2**64 - 1))
+                lib().ea_hq_init(C.byref(self.pts[i * nr + r]), s32 & (2**64 - 1))
+        self.samples = [init_sample(L, Lz, nr, nt, bc, variant, mattis,
+                                    int(self.seeds_J[i]), int(self.seeds_u[i]))
+                        for i in range(ns)]
+
+
+class Simulation:
+    """Seeds -> generators -> samples -> engine, as main() and Init() do
+    (main.cu:70-119, ini.c:8-101), then the step loop on the device."""
+
+    def __init__(self, L, Lz, nr, nt, ns, bc=PBC, betas=None, lut=None,
+                 seeds=(1, 2, 3, 4), job_id=0, variant=0, mattis=0,
+                 list_samples=None, clone_thr=None, prepared=None):
+        if betas is None:
+            raise ValueError("betas required")
+        if prepared is None:
+            prepared = Prepared(L, Lz, nr, nt, ns, bc, seeds, variant, mattis,
+                                list_samples)
+        p = prepared
+        self.L, self.Lz, self.nr, self.nt, self.ns = L, Lz, nr, nt, ns
+        self.bc, self.job = bc, job_id
+        self.V = L * L * Lz
+        self.seed_J, self.seed_u, self.seed_MC, self.seed_cl = seeds
+        self.list_samples = p.list_samples
+        self.seeds_J, self.seeds_u, self.seeds_MC = p.seeds_J, p.seeds_u, p.seeds_MC
+        self.cl, self.prc, self.xo, self.keys = p.cl, p.prc, p.xo, p.keys
         self.engine = Engine(L, Lz, nr, nt, ns, bc, betas, lut=lut,
                              clone_thr=clone_thr, job_id=job_id)
         self.engine.set_keys(self.keys)
         self.countJ = []
         self.couplings = []
-        for i in range(ns):
-            ev, od, J, cp, cnt = init_sample(L, Lz, nr, nt, bc, variant,
-                                             mattis, int(self.seeds_J[i]),
-                                             int(self.seeds_u[i]))
+        for i, (ev, od, J, cp, cnt) in enumerate(p.samples):
             self.engine.upload_sample(i, ev, od, J)
             self.countJ.append(cnt)
             self.couplings.append(cp)
-        self.engine.upload_pt_state(pts)
+        self.engine.upload_pt_state(p.pts)
         self.iter = 0
         self.rng_offset = 0
 

@@ -17,7 +17,9 @@ The reference has no counterpart (it runs one tube size per process,
 PROGRAM/Makefile:3-36); each request is that program's run of one job.
 """
 import math
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -90,21 +92,31 @@ class _Luts:
 
     def __init__(self):
         self.cache = {}
+        self.lock = threading.Lock()
 
     def get(self, r):
         key = (r.nt, r.tmin, r.tmax)
-        if key not in self.cache:
-            betas = eatube.betas_linear(r.tmin, r.tmax, r.nt)
-            self.cache[key] = (betas, eatube.make_luts(betas))
-        return self.cache[key]
+        with self.lock:
+            if key not in self.cache:
+                betas = eatube.betas_linear(r.tmin, r.tmax, r.nt)
+                self.cache[key] = (betas, eatube.make_luts(betas))
+            return self.cache[key]
 
 
-def start(r, luts):
+def prepare(r, luts):
+    """The host half of request r's start (eatube.Prepared): no GPU work,
+    so it runs on a worker thread while the GPU serves the running ones."""
+    luts.get(r)
+    return eatube.Prepared(r.L, r.Lz, r.nr, r.nt, r.ns, bc=r.bc, seeds=r.seeds())
+
+
+def start(r, luts, prepared=None):
     """Device-resident state of request r, initialised as the reference
     program does (eatube.Simulation)."""
     betas, lut = luts.get(r)
     return eatube.Simulation(r.L, r.Lz, r.nr, r.nt, r.ns, bc=r.bc, betas=betas,
-                             lut=lut, seeds=r.seeds(), job_id=r.rid % (1 << 14))
+                             lut=lut, seeds=r.seeds(), job_id=r.rid % (1 << 14),
+                             prepared=prepared)
 
 
 def footprint(r, luts):
@@ -118,25 +130,32 @@ class Server:
     chunk      steps every running request advances between admissions
     max_tubes  most requests on the GPU at once
     keep       keep each result's final state (spins, perms, energy sums)
+    workers    threads preparing arrived requests on the host (prepare_fn)
+               while the GPU serves; 0: each request is prepared inside
+               start_fn at its admission
     """
 
     def __init__(self, budget, chunk=32, max_tubes=256, keep=False,
                  start_fn=start, batch_fn=eatube.Batch, footprint_fn=footprint,
-                 clock=time.perf_counter):
+                 clock=time.perf_counter, workers=0, prepare_fn=prepare):
         self.budget = int(budget)
         self.chunk = int(chunk)
         self.max_tubes = int(max_tubes)
         self.keep = keep
         self.start_fn, self.batch_fn = start_fn, batch_fn
         self.footprint_fn, self.clock = footprint_fn, clock
+        self.workers, self.prepare_fn = int(workers), prepare_fn
         self.luts = _Luts()
         self.stats = dict(chunks=0, batches_built=0, max_running=0,
-                          max_bytes=0, deferred=0)
+                          max_bytes=0, deferred=0, t_start=0.0, t_batch=0.0,
+                          t_schedule=0.0, t_run=0.0, t_finish=0.0)
 
     def run(self, requests):
         """Serve `requests` (arrival times relative to now); returns the
         results in request order."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        pool = ThreadPoolExecutor(self.workers) if self.workers > 0 else None
+        futures = {}  # rid -> preparation in flight
         t0 = self.clock()
         running = {}  # rid -> [request, sim, steps left, bytes, admitted]
         used = 0
@@ -144,10 +163,21 @@ class Server:
         batch, members = None, ()
         while pending or running:
             now = self.clock() - t0
-            # admission: arrived requests, in arrival order, while they fit
-            while pending and pending[0].arrival <= now and \
+            if pool is not None:  # arrived requests start preparing at once
+                for r in pending:
+                    if r.arrival > now:
+                        break
+                    if r.rid not in futures:
+                        futures[r.rid] = pool.submit(self.prepare_fn, r, self.luts)
+            # admission: arrived (and prepared) requests, in arrival order,
+            # while they fit
+            k = 0
+            while k < len(pending) and pending[k].arrival <= now and \
                     len(running) < self.max_tubes:
-                r = pending[0]
+                r = pending[k]
+                if pool is not None and not futures[r.rid].done():
+                    k += 1
+                    continue
                 need = self.footprint_fn(r, self.luts)
                 if need > self.budget:
                     raise ValueError("request %d needs %d bytes, more than the "
@@ -155,32 +185,47 @@ class Server:
                 if used + need > self.budget:
                     self.stats["deferred"] += 1
                     break
-                pending.pop(0)
-                running[r.rid] = [r, self.start_fn(r, self.luts), r.sweeps,
-                                  need, self.clock() - t0]
+                pending.pop(k)
+                ts = self.clock()
+                if pool is None:
+                    sim = self.start_fn(r, self.luts)
+                else:
+                    sim = self.start_fn(r, self.luts, futures.pop(r.rid).result())
+                running[r.rid] = [r, sim, r.sweeps, need, self.clock() - t0]
+                self.stats["t_start"] += self.clock() - ts
                 used += need
             self.stats["max_running"] = max(self.stats["max_running"], len(running))
             self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
             if not running:
-                time.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
+                if pending and pending[0].arrival > self.clock() - t0:
+                    time.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
+                else:
+                    time.sleep(0.0005)  # arrived requests still preparing
                 continue
             # the running requests, ordered by size, are the batch's tubes
             order = tuple(sorted(running, key=lambda k: (
                 running[k][0].L * running[k][0].L * running[k][0].Lz, k)))
+            ts = self.clock()
             if order != members:
                 if batch is not None:
                     batch.close()
                 batch = self.batch_fn([running[k][1].engine for k in order])
                 members = order
                 self.stats["batches_built"] += 1
+            tb = self.clock()
             scheds = []
             for k in order:
                 r, sim, left = running[k][:3]
                 scheds.append(sim.schedule(min(self.chunk, left),
                                            cluster=r.cluster))
+            tc = self.clock()
             batch.run(scheds)
             self.stats["chunks"] += 1
             done = self.clock() - t0
+            self.stats["t_batch"] += tb - ts
+            self.stats["t_schedule"] += tc - tb
+            self.stats["t_run"] += self.clock() - tc
+            ts = self.clock()
             for k, sch in zip(order, scheds):
                 running[k][2] -= sch.n_it
                 if running[k][2] == 0:
@@ -195,8 +240,11 @@ class Server:
                     sim.engine.close()
                     used -= need
                     results[r.rid] = res
+            self.stats["t_finish"] += self.clock() - ts
         if batch is not None:
             batch.close()
+        if pool is not None:
+            pool.shutdown()
         return [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
 
 

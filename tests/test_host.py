@@ -195,3 +195,12 @@ def test_pt_seeds_are_sign_extended_int():
             eatube.lib().ea_hq_init(C.byref(pts[i * nr + r]),
                                     s32 & (2**64 - 1))
     g.check("pt_init", bytes(pts))
+
+
+def test_seed_prefix_property():
+    """generate_seeds_from_one emits seeds in sequence: asking for fewer
+    gives the same leading seeds (the engine makes only those it uses)."""
+    for seed in (1, 987654321, 2**63 + 11):
+        full = eatube.seeds_from_one(seed, 32)
+        for k in (1, 3, 17):
+            assert np.array_equal(eatube.seeds_from_one(seed, k), full[:k])

@@ -136,3 +136,39 @@ def test_shape_signature_and_cache(tmp_path):
     assert eatube.tune_shape([b, a], cache=cache) == ((32, 512, 0, 1), {})
     assert eatube.DEFAULT_SHAPE in eatube.SHAPES
     assert len(set(eatube.SHAPES)) == len(eatube.SHAPES) == 18
+
+
+def test_worker_preparation_serves_every_request_once():
+    """With host preparation on worker threads, requests are admitted only
+    once prepared, every step is served exactly once, and each start gets
+    the request's own preparation."""
+    clock, log, sims, prepared = Clock(), [], {}, {}
+
+    def prepare(r, luts):
+        prepared[r.rid] = ("prep", r.rid)
+        return prepared[r.rid]
+
+    def start(r, luts, prep):
+        assert prep == ("prep", r.rid)
+        sims[r.rid] = FakeSim(r, log)
+        return sims[r.rid]
+
+    class Batch:
+        def __init__(self, engines):
+            log.append(("batch", tuple(e.rid for e in engines)))
+
+        def run(self, scheds):
+            clock.t += 0.01 * max(s.n_it for s in scheds)
+
+        def close(self):
+            pass
+
+    srv = serve.Server(10 ** 9, chunk=7, start_fn=start, batch_fn=Batch,
+                       footprint_fn=lambda r, luts: 1, clock=clock, workers=3,
+                       prepare_fn=prepare)
+    reqs = [req(i, 8 * (1 + i % 3), 10 + 7 * i, 0.05 * i) for i in range(9)]
+    res = srv.run(reqs)
+    assert [r.rid for r in res] == list(range(9))
+    assert all(sims[r.rid].steps == r.sweeps for r in reqs)
+    assert set(prepared) == set(range(9))
+    assert all(x.finished >= x.admitted >= x.arrival for x in res)
