@@ -8,6 +8,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -680,7 +681,8 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
   __shared__ uint64_t ira[256];
   __shared__ uint64_t zs[EA_MAX_BETAS], prs[EA_MAX_BETAS];
   __shared__ double uu[EA_MAX_BETAS], lu[EA_MAX_BETAS];
-  __shared__ int32_t E[EA_MAX_BETAS];
+  __shared__ int32_t E[EA_MAX_BETAS];   // by clone
+  __shared__ int32_t Ep[EA_MAX_BETAS];  // by position (beta) before the swaps
   __shared__ uint8_t perm[EA_MAX_BETAS + 1];
   __shared__ int32_t dE[EA_MAX_BETAS];
   __shared__ uint8_t took[EA_MAX_BETAS];
@@ -688,6 +690,8 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
   __shared__ double sacc[EA_MAX_BETAS], satt[EA_MAX_BETAS], sraw[EA_MAX_BETAS],
       sesum[EA_MAX_BETAS];
   __shared__ int32_t stlen[EA_MAX_BETAS], stoff[EA_MAX_BETAS];
+  __shared__ int32_t dmin[EA_MAX_BETAS];  // pair i swaps iff its energy step >= dmin[i]
+  __shared__ int exact_walk;
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
@@ -701,7 +705,7 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
   // everything the serial pass reads is staged before it: no global load
   // sits on the chain of dependent swap decisions
   const uint64_t zseed = st->zseed;
-  const uint8_t ip = st->ip;
+  const uint8_t ip = st->ip, ip1 = st->ip1, ip2 = st->ip2, ip3 = st->ip3;
   for (int i = lane; i < 256; i += 64) ira[i] = st->ira[i];
   for (int i = lane; i < nt; i += 64) perm[i] = wctb[i];
   const size_t k0 = (size_t)chain * np;
@@ -729,7 +733,9 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
     __syncthreads();
   } else {
     for (int c = lane; c < nt; c += 64) E[c] = ener[c];
+    __syncthreads();
   }
+  for (int p = lane; p < nt; p += 64) Ep[p] = E[perm[p]];
   if (lane == 0) {  // the congruential half is a cheap serial recurrence
     uint64_t z = zseed;
     for (int k = 0; k < np; ++k) zs[k] = z = z * kLcg + 1;
@@ -753,6 +759,7 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
     }
     __syncthreads();
   }
+  if (lane == 0) exact_walk = 0;
   for (int k = lane; k < np; k += 64) {
     const uint64_t fin = zs[k] + prs[k];
     const double u = kFnorm * (double)fin;
@@ -765,30 +772,73 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
     }
   }
   __syncthreads();
+  // Each pair's decision is monotone in its (even) energy step d: pt grows
+  // with d, and both the log comparison and the exact one against the
+  // table of exp values accept more as pt grows.  So pair i swaps iff
+  // d >= dmin[i], the smallest even d its own decision rule accepts, found
+  // by evaluating that rule around log(u)/nd; a threshold that does not
+  // verify (rule true at dmin, false at dmin - 2) sends the walk back to
+  // the rule itself.
+  for (int k = lane; k < np; k += 64) {
+    auto take = [&](int d) {
+      const double pt = ndv[k] * (double)d;
+      if (pt >= 0) return true;
+      const double u = uu[k], l = lu[k];
+      if (u == 0.0 || fabs(l - pt) <= 1e-12) return u < exact_exp(a, k, d);
+      return l < pt;
+    };
+    int dm;
+    if (!(ndv[k] > 0.0)) {
+      dm = INT_MIN;  // pt = 0: always swap
+    } else if (uu[k] == 0.0) {
+      dm = -2 * stlen[k];  // u = 0 < exp(pt) while the table has a value
+    } else {
+      const double x = lu[k] / ndv[k];  // the rule turns near d = x < 0
+      dm = x < -1e9 ? INT_MIN : 2 * (int)floor(x * 0.5) - 4;
+      for (int t = 0; t < 8 && dm != INT_MIN && !take(dm); ++t) dm += 2;
+    }
+    if (dm > 0) dm = 0;
+    if (dm != INT_MIN && (!take(dm) || take(dm - 2))) exact_walk = 1;
+    dmin[k] = dm;
+  }
+  __syncthreads();
   if (lane == 0) {
+    // the clone moving up the ladder and its energy stay in registers:
+    // position i+1 still holds its clone from before the walk, so no LDS
+    // access depends on an earlier decision
+    int pc = perm[0];
+    int32_t ec = Ep[0];
     for (int i = 0; i < np; ++i) {
-      const int d = E[perm[i + 1]] - E[perm[i]];
-      const double pt = ndv[i] * (double)d;
-      bool take = true;
-      if (!(pt >= 0)) {
-        const double u = uu[i], l = lu[i];
-        if (u == 0.0 || fabs(l - pt) <= 1e-12)
-          take = u < exact_exp(a, i, d);
-        else
-          take = l < pt;
+      const int pb = perm[i + 1];
+      const int32_t eb = Ep[i + 1];
+      const int d = eb - ec;
+      bool take = d >= dmin[i];
+      if (exact_walk) {
+        const double pt = ndv[i] * (double)d;
+        take = true;
+        if (!(pt >= 0)) {
+          const double u = uu[i], l = lu[i];
+          if (u == 0.0 || fabs(l - pt) <= 1e-12)
+            take = u < exact_exp(a, i, d);
+          else
+            take = l < pt;
+        }
       }
       dE[i] = d;
       took[i] = take;
-      if (take) {
-        const uint8_t tmp = perm[i];
-        perm[i] = perm[i + 1];
-        perm[i + 1] = tmp;
+      if (take) {  // pc moves to position i+1
+        perm[i] = (uint8_t)pb;
+      } else {
+        perm[i] = (uint8_t)pc;
+        pc = pb;
+        ec = eb;
       }
     }
+    perm[np] = (uint8_t)pc;
     st->ip = (uint8_t)(ip + np);
-    st->ip1 = (uint8_t)(st->ip1 + np);
-    st->ip2 = (uint8_t)(st->ip2 + np);
-    st->ip3 = (uint8_t)(st->ip3 + np);
+    st->ip1 = (uint8_t)(ip1 + np);
+    st->ip2 = (uint8_t)(ip2 + np);
+    st->ip3 = (uint8_t)(ip3 + np);
   }
   __syncthreads();
   for (int i = lane; i < 256; i += 64) st->ira[i] = ira[i];
