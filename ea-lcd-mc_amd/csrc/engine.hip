@@ -697,10 +697,10 @@ static int check_shape(const ea_shape *sh) {
   if (sh->cluster_threads != 256 && sh->cluster_threads != 512 && sh->cluster_threads != 1024)
     return fail(EA_E_ARG, "cluster_threads must be 256, 512 or 1024 (got %d)",
                 sh->cluster_threads);
-  if (sh->sweep_kind != 0 && sh->sweep_kind != 1)
-    return fail(EA_E_ARG, "sweep_kind must be 0 or 1 (got %d)", sh->sweep_kind);
-  if (sh->sweep_kind == 1 && sh->sweep_positions != 64)
-    return fail(EA_E_ARG, "sweep_kind 1 takes sweep_positions 64");
+  if (sh->sweep_kind < 0 || sh->sweep_kind > 2)
+    return fail(EA_E_ARG, "sweep_kind must be 0, 1 or 2 (got %d)", sh->sweep_kind);
+  if (sh->sweep_kind != 0 && sh->sweep_positions != 64)
+    return fail(EA_E_ARG, "sweep_kind %d takes sweep_positions 64", sh->sweep_kind);
   if (sh->sweep_xcd != 0 && sh->sweep_xcd != 1)
     return fail(EA_E_ARG, "sweep_xcd must be 0 or 1 (got %d)", sh->sweep_xcd);
   return EA_OK;
@@ -720,6 +720,8 @@ static Shape to_shape(const ea_shape *sh) {
 extern "C" int ea_set_shape(ea_engine *e, const ea_shape *shape) {
   if (!e) return fail(EA_E_ARG, "NULL engine");
   RC_TRY(check_shape(shape));
+  if (!shape_fits(e->g(), to_shape(shape)))
+    return fail(EA_E_ARG, "sweep_kind 2: this tube's tile (256 + L*L/16 half sites) exceeds 768");
   if (e->part_pending) RC_TRY(ensure_energy(e));
   e->shape = to_shape(shape);
   e->t.nbx = sweep_partials_x(e->g(), e->shape);
@@ -908,6 +910,10 @@ static int drop_graph(ea_batch *b) {
 extern "C" int ea_batch_set_shape(ea_batch *b, const ea_shape *shape) {
   if (!b) return fail(EA_E_ARG, "NULL batch");
   RC_TRY(check_shape(shape));
+  for (size_t i = 0; i < b->eng.size(); ++i)
+    if (!shape_fits(b->eng[i]->g(), to_shape(shape)))
+      return fail(EA_E_ARG, "sweep_kind 2: tube %zu's tile (256 + L*L/16 half sites) exceeds 768",
+                  i);
   b->launch = to_shape(shape);
   return EA_OK;
 }

@@ -569,6 +569,175 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_lane(const Tube *__restr
   }
 }
 
+// Workgroup barrier that waits for this wave's LDS writes only: global loads
+// issued before it stay in flight (__syncthreads waits for all memory
+// traffic).  The signal fences keep the compiler from moving memory
+// accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt, expcnt untouched
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// One half-sweep, kind 2 (tiled): blocks as kind 1 (256 consecutive half
+// sites of one replica at one beta), but a sample's six neighbour words come
+// from an LDS tile of the opposite parity's clone row over the block's run
+// widened by hs on each side (every x, y and unwrapped z neighbour): each
+// thread loads K <= 3 tile words per sample instead of six neighbours.
+// Samples go in pairs; the next pair's tiles are loaded with this pair's
+// words and staged after the update (double buffer, barriers that leave
+// global traffic in flight).  All loads are unconditional (clamped
+// addresses), so they stay ahead of the one wait of the update instead of
+// sinking towards their use.  Only lanes whose z neighbour wraps around the
+// tube read it from global memory.
+constexpr int kTileWordsMax = 3;  // tile of at most 768 half sites
+
+__host__ __device__ inline int tile_width(const Lattice &g) { return kLaneThreads + 2 * g.hs; }
+inline int tile_words(const Lattice &g) {
+  return (tile_width(g) + kLaneThreads - 1) / kLaneThreads;
+}
+
+template <int K, bool XCD>
+__global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restrict__ tubes,
+                                                             const uint64_t *__restrict__ map,
+                                                             Phase ph) {
+  __shared__ __align__(16) uint32_t lut[EA_LUT_BYTES / 4];
+  __shared__ uint32_t clone_off[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
+  __shared__ uint32_t ctr[EA_MAX_REPLICAS][5];
+  __shared__ uint64_t tile[2][2][K * kLaneThreads];  // [buffer][sample of the pair][x]
+  const Where wh = locate_block(tubes, map, ph.step,
+                                XCD ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const SweepBlock b = sweep_block(g, wh.local);
+  const int tx = threadIdx.x, r = b.r;
+  sweep_tables<true>(a, b.beta, wh.step, ph.half, lut, clone_off, ctr);
+  __syncthreads();
+
+  const int h0 = b.hb * kLaneThreads, hh = h0 + tx;
+  const bool valid = hh < g.n;
+  const int h = valid ? hh : 0;
+  uint64_t id1 = 0, id2 = 0;
+  if (valid) {
+    uint32_t w[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = sweep_word(g, ctr, lut, b.beta, h, r, k);
+    thresholds(w, id1, id2);
+  }
+  const int p = ph.half & 1;
+  const LaneSite ls = lane_site(g, p, h);
+  const uint32_t opp = (uint32_t)((1 - p) * g.half);
+  const int W = tile_width(g), lo = h0 - g.hs;  // the tile holds half sites [lo, lo + W)
+  int ti[6];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    const int x = (int)(ls.nb[d] - opp) - lo;
+    ti[d] = x >= 0 && x < W ? x : -1;  // -1: a z neighbour across the tube's wrap
+  }
+  // this thread's tile slots (clamped: a slot past the tile or the tube
+  // repeats a word nobody reads)
+  int tx_[K];
+  uint32_t th_[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    tx_[i] = min(tx + i * kLaneThreads, W - 1);
+    th_[i] = (uint32_t)min(max(lo + tx_[i], 0), g.n - 1);
+  }
+  uint64_t *const spin = a.spin;
+  const uint64_t *const J = a.J;
+  int32_t *const part = a.part;
+  const int nbx = a.nbx, bx = b.hb * (kLaneThreads / 64) + (tx >> 6);
+  const uint32_t pstride = (uint32_t)(g.nr * g.nt * nbx);
+  const uint32_t pbase = (uint32_t)((r * g.nt + b.beta) * nbx + bx);
+  const bool pwrite = (tx & 63) == 0 && bx < nbx;
+  const size_t sstride = 2 * (size_t)g.half, jstride = 12 * (size_t)g.n;
+  const uint32_t jpair = 2 * (uint32_t)g.n;
+
+  uint64_t tv[2][K];
+  auto fetch = [&](int slot, int s) {
+    const gword *S = gmem(spin + s * sstride) + clone_off[s * g.nr + r] + opp;
+#pragma unroll
+    for (int i = 0; i < K; ++i) tv[slot][i] = S[th_[i]];
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      tile[buf][0][tx_[i]] = tv[0][i];
+      tile[buf][1][tx_[i]] = tv[1][i];
+    }
+  };
+
+  const int last = g.ns - 1;
+  fetch(0, 0);
+  fetch(1, min(1, last));
+  stage(0);
+  lds_barrier();
+  for (int s0 = 0, buf = 0; s0 < g.ns; s0 += 2, buf ^= 1) {
+    const int s1 = min(s0 + 1, last);
+    const uint32_t c0 = clone_off[s0 * g.nr + r], c1 = clone_off[s1 * g.nr + r];
+    gword *S0 = gmem(spin + s0 * sstride), *S1 = gmem(spin + s1 * sstride);
+    fetch(0, min(s0 + 2, last));  // the next pair's tiles (after the last
+    fetch(1, min(s0 + 3, last));  // pair: repeats, not staged)
+    SiteIn x0, x1;
+    const gword *J0 = gmem(J + s0 * jstride), *J1 = gmem(J + s1 * jstride);
+    x0.sv = S0[c0 + ls.own];
+    x1.sv = S1[c1 + ls.own];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      x0.J[2 * k] = J0[ls.jo + k * jpair];
+      x0.J[2 * k + 1] = J0[ls.jo + k * jpair + 1];
+      x1.J[2 * k] = J1[ls.jo + k * jpair];
+      x1.J[2 * k + 1] = J1[ls.jo + k * jpair + 1];
+    }
+    uint64_t z0[2] = {0, 0}, z1[2] = {0, 0};
+    if (ti[2] < 0) {
+      z0[0] = S0[c0 + ls.nb[2]];
+      z1[0] = S1[c1 + ls.nb[2]];
+    }
+    if (ti[5] < 0) {
+      z0[1] = S0[c0 + ls.nb[5]];
+      z1[1] = S1[c1 + ls.nb[5]];
+    }
+    int e0 = 0, e1 = 0;
+    uint64_t n0 = 0, n1 = 0;
+    if (valid) {
+#pragma unroll
+      for (int d = 0; d < 6; ++d) {
+        x0.sc[d] = ti[d] >= 0 ? tile[buf][0][ti[d]] : 0;
+        x1.sc[d] = ti[d] >= 0 ? tile[buf][1][ti[d]] : 0;
+      }
+      if (ti[2] < 0) {
+        x0.sc[2] = z0[0];
+        x1.sc[2] = z1[0];
+      }
+      if (ti[5] < 0) {
+        x0.sc[5] = z0[1];
+        x1.sc[5] = z1[1];
+      }
+      fix_lane(ls, p, x0);
+      fix_lane(ls, p, x1);
+      n0 = x0.sv ^ site_flip(x0, id1, id2, ph.energy, e0);
+      n1 = x1.sv ^ site_flip(x1, id1, id2, ph.energy, e1);
+    }
+    stage(buf ^ 1);  // after the last pair: into a buffer nobody reads
+    if (valid) {
+      S0[c0 + ls.own] = n0;
+      if (s1 != s0) S1[c1 + ls.own] = n1;
+    }
+    if (ph.energy) {
+      e0 = group_sum<64>(e0);
+      e1 = group_sum<64>(e1);
+      if (pwrite) {
+        part[(uint32_t)s0 * pstride + pbase] = e0;
+        if (s1 != s0) part[(uint32_t)s1 * pstride + pbase] = e1;
+      }
+    }
+    lds_barrier();
+  }
+}
+
 // ----------------------------------------------------------------- energy
 
 // Energy of every (sample, replica, clone) from the even words
@@ -1512,6 +1681,10 @@ int sweep_partials_x(const Lattice &g, const Shape &shape) {
   return (int)blocks_for(g.n, shape.sweep_kind ? 64 : shape.sweep_positions);
 }
 
+bool shape_fits(const Lattice &g, const Shape &shape) {
+  return shape.sweep_kind != 2 || tile_words(g) <= kTileWordsMax;
+}
+
 static long long sweep_grid(const Tube &t, const Shape &shape) {
   const Lattice &g = t.g;
   const long long per = shape.sweep_kind ? blocks_for(g.n, kLaneThreads) : t.nbx;
@@ -1550,6 +1723,7 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
       for (long long i = 0; i < count; ++i) p.map[kind].push_back(tag | (uint64_t)i);
     };
     add(kMapSweep, sweep_grid(t, shape));
+    if (shape.sweep_kind == 2) p.tile_words = std::max(p.tile_words, tile_words(g));
     add(kMapEnergy, (long long)energy_blocks_host(g) * g.nt * g.ns);
     add(kMapChain, (long long)g.ns * g.nr);
     if (!t.cluster || g.ctm <= 0) continue;
@@ -1592,6 +1766,7 @@ hipError_t upload_plan(const Plan &p, DevPlan *d) {
   d->lds32 = p.lds32;
   d->lds64 = p.lds64;
   d->lds_slab = p.lds_slab;
+  d->tile_words = p.tile_words;
   d->shape = p.shape;
   d->identity = p.ntubes == 1;
   if (!total || d->identity) return hipSuccess;
@@ -1634,7 +1809,18 @@ hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream
   const dim3 grid((unsigned)p.cnt[kMapSweep]);
   const uint64_t *m = p.map(kMapSweep);
   const Shape &sh = p.shape;
-  if (sh.sweep_kind == 1) {
+  if (sh.sweep_kind == 2) {
+    const dim3 blk(kLaneThreads);
+    if (p.tile_words <= 1)
+      hipLaunchKernelGGL((sh.sweep_xcd ? k_sweep_tile<1, true> : k_sweep_tile<1, false>), grid,
+                         blk, 0, s, tubes, m, ph);
+    else if (p.tile_words == 2)
+      hipLaunchKernelGGL((sh.sweep_xcd ? k_sweep_tile<2, true> : k_sweep_tile<2, false>), grid,
+                         blk, 0, s, tubes, m, ph);
+    else
+      hipLaunchKernelGGL((sh.sweep_xcd ? k_sweep_tile<3, true> : k_sweep_tile<3, false>), grid,
+                         blk, 0, s, tubes, m, ph);
+  } else if (sh.sweep_kind == 1) {
     if (sh.sweep_xcd)
       hipLaunchKernelGGL(k_sweep_lane<true>, grid, dim3(kLaneThreads), 0, s, tubes, m, ph);
     else

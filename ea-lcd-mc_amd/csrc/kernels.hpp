@@ -118,6 +118,7 @@ inline __host__ __device__ bool measured_in_pt(const Tube &t) {
 // among the samples through LDS (1024-thread blocks of sweep_positions = 32
 // or 64 half sites of one replica), kind 1 gives each thread one position
 // and all samples (256-thread blocks, one energy partial per wave);
+// kind 2 is kind 1 with the neighbour words staged in LDS tiles;
 // sweep_xcd = 1 hands each XCD a contiguous range of sweep blocks.  cluster_threads: threads of
 // a whole-tube Houdayer workgroup (256, 512 or 1024).
 struct Shape {
@@ -136,6 +137,7 @@ struct Plan {
   std::vector<uint64_t> map[kMapKinds];
   Shape shape;
   size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0;
+  int tile_words = 0;  // tile words per thread of a tiled sweep (the widest tube's)
   int ntubes = 0;
 };
 
@@ -147,6 +149,7 @@ struct DevPlan {
   size_t off[kMapKinds] = {};
   size_t cnt[kMapKinds] = {};
   size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0;
+  int tile_words = 0;  // tile words per thread of a tiled sweep (the widest tube's)
   Shape shape;
   bool identity = false;
   const uint64_t *map(int k) const { return identity ? nullptr : buf + off[k]; }
@@ -156,6 +159,9 @@ struct DevPlan {
 // Energy partials per (sample, replica, beta) a sweep of this shape writes
 // (a tube's nbx), and the partial-energy slots a tube needs under any shape.
 int sweep_partials_x(const Lattice &g, const Shape &shape);
+// Whether a tube can take a launch shape (the tiled sweep needs its tile,
+// 256 + 2 hs half sites, within 768).
+bool shape_fits(const Lattice &g, const Shape &shape);
 size_t energy_partials(const Lattice &g);
 
 // Block maps of the tubes (host copies of the descriptors give geometry,

@@ -535,7 +535,7 @@ class Simulation:
 # sweep_kind, sweep_xcd): the sweep options, and the Houdayer workgroup
 # sizes.
 SWEEP_OPTIONS = ((64, 0, 0), (32, 0, 0), (64, 0, 1), (32, 0, 1), (64, 1, 0),
-                 (64, 1, 1))
+                 (64, 1, 1), (64, 2, 0), (64, 2, 1))
 CLUSTER_THREADS = (1024, 512, 256)
 DEFAULT_SHAPE = (64, 1024, 0, 0)
 SHAPES = tuple((p, c, k, x) for (p, k, x) in SWEEP_OPTIONS
@@ -594,7 +594,10 @@ def tune_shape(engines, ptfr=1, houfr=1, mesfr=1, cluster=True, steps=8,
     times = {}
 
     def timed(shp):
-        batch.set_shape(*shp)
+        try:
+            batch.set_shape(*shp)
+        except EngineError:
+            return  # a tube of the mix cannot take this shape
         batch.run([t.schedule(1, **kw) for t in twins])
         sch = [t.schedule(steps, **kw) for t in twins]
         t0 = time.perf_counter()

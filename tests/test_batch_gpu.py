@@ -133,7 +133,7 @@ def test_every_launch_shape_gives_the_same_state():
         solo.engine.set_shape(*shp)
         solo.steps(1, cluster=True)
         solo.engine.sweep(solo.counters(2))      # leaves partials pending
-        other = (64, shp[1], 1 - shp[2], 1 - shp[3])
+        other = (64, shp[1], (shp[2] + 1) % 3, 1 - shp[3])
         solo.engine.set_shape(*other)            # across a pending sweep
         solo.engine.pt()
         p = make(specs[1], 31)
@@ -161,3 +161,24 @@ def test_tuner_picks_a_shape_and_caches_it(tmp_path):
     again, t2 = eatube.tune_shape([s.engine for s in sims[::-1]],
                                   cache=eatube.load_shape_cache(path))
     assert again == best and t2 == {}
+
+
+@pytest.mark.parametrize("nr", [2, 8, 16])
+def test_launch_shapes_with_other_replica_counts(nr):
+    """The sweep kinds with 2, 8 and 16 replicas (block maps, partials and
+    energy sums by replica) equal the default shape, with cluster moves."""
+    L, Lz, bc, nt, ns = 16, 32, 1, 4, 3
+    betas = eatube.betas_linear(0.6, 1.6, nt)
+
+    def sim(shape):
+        x = eatube.Simulation(L, Lz, nr, nt, ns, bc=bc, betas=betas,
+                              seeds=(71, 72, 73, 74), job_id=5)
+        if shape:
+            x.engine.set_shape(*shape)
+        x.steps(2, cluster=True)
+        return state(x)
+
+    want = sim(None)
+    for shp in ((32, 512, 0, 1), (64, 1024, 1, 0), (64, 256, 2, 1)):
+        say("nr", nr, "shape", shp)
+        same(want, sim(shp), "nr %d shape %s" % (nr, shp))
