@@ -166,7 +166,8 @@ def test_tuner_picks_a_shape_and_caches_it(tmp_path):
 @pytest.mark.parametrize("nr", [2, 8, 16])
 def test_launch_shapes_with_other_replica_counts(nr):
     """The sweep kinds with 2, 8 and 16 replicas (block maps, partials and
-    energy sums by replica) equal the default shape, with cluster moves."""
+    energy sums by replica) equal the default shape (with cluster moves
+    where there are two replicas per half)."""
     L, Lz, bc, nt, ns = 16, 32, 1, 4, 3
     betas = eatube.betas_linear(0.6, 1.6, nt)
 
@@ -175,7 +176,7 @@ def test_launch_shapes_with_other_replica_counts(nr):
                               seeds=(71, 72, 73, 74), job_id=5)
         if shape:
             x.engine.set_shape(*shape)
-        x.steps(2, cluster=True)
+        x.steps(2, cluster=nr >= 4)  # a Houdayer pair needs 2 replicas per half
         return state(x)
 
     want = sim(None)
