@@ -1311,25 +1311,37 @@ __device__ void label_rows(const Lattice &g, I *lab, const uint64_t *rows, int p
     lab[i] = r * L + R::rep(R::bounds(rows[r], L, full), x);
   }
   __syncthreads();
-  for (int r = tid; r < nrows; r += T) {
-    const uint64_t q = rows[r];
-    const uint64_t lq = ~(q ^ (q << 1)) & full & ~1ULL;  // q[x] == q[x-1]
-    const int zl = qdiv(r, g.mL), y = r - zl * L;
-    auto link = [&](int r2) {
-      const uint64_t q2 = rows[r2];
-      const uint64_t l2 = ~(q2 ^ (q2 << 1)) & full & ~1ULL;
-      uint64_t todo = ~(q ^ q2) & full & ~(lq & l2);
-      while (todo) {
-        const int x = R::lo(todo);
-        todo &= todo - 1;
-        f.unite(r * L + x, r2 * L + x);
+  {
+    // Links: id = r + nrows * dir joins row r to row r+1 in y (dir 0) or to
+    // the next plane (dir 1), taken id = tid, tid + T, ... in one loop.
+    uint64_t todo = 0;
+    int base = 0, base2 = 0;
+    for (int id = tid;;) {
+      while (!todo && id < 2 * nrows) {
+        const int dir = id >= nrows, r = id - dir * nrows;
+        id += T;
+        const int zl = qdiv(r, g.mL), y = r - zl * L;
+        int r2;
+        if (dir == 0)
+          r2 = y + 1 == L ? r - (L - 1) : r + 1;
+        else if (zl + 1 < planes)
+          r2 = r + L;
+        else if (wrap_z)
+          r2 = r - L * (planes - 1);
+        else
+          continue;
+        const uint64_t q = rows[r], q2 = rows[r2];
+        const uint64_t lq = ~(q ^ (q << 1)) & full & ~1ULL;  // q[x] == q[x-1]
+        const uint64_t l2 = ~(q2 ^ (q2 << 1)) & full & ~1ULL;
+        todo = ~(q ^ q2) & full & ~(lq & l2);
+        base = r * L;
+        base2 = r2 * L;
       }
-    };
-    link(y + 1 == L ? r - (L - 1) : r + 1);
-    if (zl + 1 < planes)
-      link(r + L);
-    else if (wrap_z)
-      link(r - L * (planes - 1));
+      if (!todo) break;
+      const int x = R::lo(todo);
+      todo &= todo - 1;
+      f.unite(base + x, base2 + x);
+    }
   }
   __syncthreads();
   for (int i = tid; i < nrows * L; i += T) lab[i] = f.root(i);

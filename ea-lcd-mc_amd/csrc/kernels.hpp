@@ -60,8 +60,8 @@ inline void set_divisors(Lattice &g) {
 // offset coin0 + (k * calls + call) * coin_per_call.
 struct Tube {
   Lattice g;
-  uint64_t *spin;           // [ns][2][nt][n][nr]
-  const uint64_t *J;        // [ns][2][n][6]
+  uint64_t *spin;           // [ns][2][nr][nt][n] (spin_row)
+  const uint64_t *J;        // [ns][2][3][n][2] (coupling_index)
   uint8_t *wctb;            // [ns][nr][nt] clone at beta
   const uint8_t *lut;       // [nt][EA_LUT_BYTES]
   int32_t *ener;            // [ns][nr][nt] by clone

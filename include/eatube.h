@@ -242,6 +242,9 @@ int ea_batch_run(ea_batch *b, const ea_schedule *schedules, uint64_t *rng_offset
  *     1024-thread block;
  *   sweep_kind 1: one thread per position updates every sample
  *     (sweep_positions must be 64);
+ *   sweep_kind 2: kind 1 with each sample's neighbour words staged in LDS
+ *     tiles (sweep_positions must be 64; the tube's tile, 256 + 2*L*L/32
+ *     half sites, must fit 768, else EA_E_ARG);
  *   sweep_xcd 1: each XCD runs a contiguous range of sweep blocks;
  *   cluster_threads: threads of a whole-tube Houdayer workgroup (256, 512
  *     or 1024).
