@@ -363,8 +363,14 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
     HIP_TRY(hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, dev));
     const int lds_max = std::max(lds_block, lds_optin);
     if (hou_whole_lds(g) > (size_t)lds_max) {
+      // slabs as thick as fit two labelling workgroups into one CU's LDS
+      // (two 1024-thread workgroups fill a CU's wave slots)
+      int lds_cu = 0;
+      HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                    dev));
+      const size_t lds_slab = std::min<size_t>(lds_max, std::max(lds_cu / 2, 1 << 15));
       int planes = std::max(1, std::min(g.Lz, kHouSlabSites / (g.L * g.L)));
-      while (planes > 1 && hou_slab_lds(g, planes) > (size_t)lds_max) --planes;
+      while (planes > 1 && hou_slab_lds(g, planes) > lds_slab) --planes;
       t.planes = planes;
       const size_t inst = (size_t)2 * g.ns * g.ctm;
       if ((rc = dalloc(&t.hlab, inst * g.V)) || (rc = dalloc(&t.hcb, inst * (g.V / 32))))
