@@ -120,8 +120,8 @@ struct ea_engine {
     free_plan(&plan);
     const void *ptrs[] = {t.spin, t.J,    t.wctb, t.lut,  t.ener, t.part,     t.pt,
                           t.acc,  t.att,  t.raw,  t.nd,   t.tab,  t.toff,     t.tlen,
-                          t.esum, t.hist, t.hlab, t.hcb,  t.step, t.ctr,      t.pairs,
-                          keep_buf, dt};
+                          t.esum, t.hist, t.hpar, t.hl16, t.hrow, t.hflag,    t.step,
+                          t.ctr,  t.pairs, keep_buf, dt};
     for (const void *p : ptrs)
       if (p) (void)hipFree(const_cast<void *>(p));
     if (stream) (void)hipStreamDestroy(stream);
@@ -369,11 +369,12 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
       HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
                                     dev));
       const size_t lds_slab = std::min<size_t>(lds_max, std::max(lds_cu / 2, 1 << 15));
-      int planes = std::max(1, std::min(g.Lz, kHouSlabSites / (g.L * g.L)));
-      while (planes > 1 && hou_slab_lds(g, planes) > lds_slab) --planes;
-      t.planes = planes;
+      t.planes = hou_slab_planes(g, lds_slab);
       const size_t inst = (size_t)2 * g.ns * g.ctm;
-      if ((rc = dalloc(&t.hlab, inst * g.V)) || (rc = dalloc(&t.hcb, inst * (g.V / 32))))
+      const size_t slabs = (g.Lz + t.planes - 1) / t.planes;
+      if ((rc = dalloc(&t.hpar, inst * g.V)) || (rc = dalloc(&t.hl16, inst * g.V)) ||
+          (rc = dalloc(&t.hrow, inst * (g.V / g.L))) ||
+          (rc = dalloc(&t.hflag, inst * slabs * 2 * hou_slab_words(g, t.planes))))
         return rc;
     }
   }
@@ -403,10 +404,8 @@ static uint64_t engine_bytes(const ea_config *cfg, int lds_max) {
                sizeof(int) + sizeof(Tube);
   const ExpTables x = make_exp_tables(cfg->betas, g.nt, g.V);
   b += 8ull * x.nd.size() + 8ull * x.off.size() + 8ull * std::max<size_t>(x.val.size(), 1);
-  if (g.ctm > 0 && hou_whole_lds(g) > (size_t)lds_max) {
-    const uint64_t inst = 2ull * g.ns * g.ctm;
-    b += 4ull * inst * g.V + 4ull * inst * (g.V / 32);
-  }
+  if (g.ctm > 0 && hou_whole_lds(g) > (size_t)lds_max)
+    b += hou_slab_bytes(g, hou_slab_planes(g, lds_max / 2));
   return b;
 }
 

@@ -1134,17 +1134,6 @@ __device__ __forceinline__ Pair pair_words(const Lattice &g, const Tube &a, cons
   return q;
 }
 
-// Linear site index (x + L y + L^2 z) of copy 0 of the parity-p site at half
-// index h; copy `bit` adds copy_offset(bit).
-__device__ __forceinline__ int site_base(const Lattice &g, int p, int h) {
-  const int gz = qdiv(h, g.mhs), rem = h - gz * g.hs, by = qdiv(rem, g.mhl);
-  const int ax = 2 * (rem - by * g.hl) + ((p + by + gz) & 1);
-  return ax + g.L * (by + g.L * gz);
-}
-__device__ __forceinline__ int copy_offset(const Lattice &g, int bit) {
-  return (bit & 3) * g.ML + g.L * (((bit >> 2) & 3) * g.ML + g.L * ((bit >> 4) * g.MLz));
-}
-
 __device__ __forceinline__ uint32_t find_root(const uint32_t *lab, uint32_t a) {
   const volatile uint32_t *v = lab;
   uint32_t p = v[a];
@@ -1559,11 +1548,21 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   }
 }
 
-// Tiled Houdayer for tubes larger than one workgroup's LDS.  Pass 1 labels
-// each slab of `planes` z-planes in LDS and stores global labels (each
-// site's slab-local root); pass 2 unites across slab boundaries in global
-// memory; pass 3 finds every site's root and applies the move.  Slab block
-// local index: slab + nslabs * instance.
+// Houdayer for tubes larger than one workgroup's LDS, in slabs of `planes`
+// z-planes; slab block local index: slab + slabs * instance.
+//  1. k_hou_slab labels a slab in LDS (label_rows) and stores every site's
+//     slab-local root (16 bits) and the q bits of every row.  A cluster
+//     that reaches neither of the slab's seam planes ends there: its label
+//     is gbase + its local root, and its flip bit is drawn at once.  The
+//     roots of the clusters that reach a seam plane become nodes of a
+//     forest in HBM (parent = itself).
+//  2. k_hou_seam unites, across each seam, the nodes of the two slabs' runs
+//     that touch in z.
+//  3. k_hou_apply gives each seam root the coin of its root in that forest,
+//     builds the move mask of every row of the slab run by run, and applies
+//     it to the slab's 16-bit pieces of the two replicas' spin words (piece
+//     bz of a word holds its sites in z block bz, and each slab owns the
+//     pieces of its planes, so blocks never write the same bits).
 __device__ __forceinline__ size_t instance_index(const Lattice &g, const Inst &in) {
   return ((size_t)in.grp * g.ns + in.s) * g.ctm + in.beta;
 }
@@ -1572,7 +1571,27 @@ __device__ __forceinline__ int slab_count(const Tube &a) {
   return (a.g.Lz + a.planes - 1) / a.planes;
 }
 
-// Dynamic LDS: rows[planes*L] u64 | lab[planes*L*L] u32.
+// coin of site index idx of the call: bit idx mod 128 of the Philox block
+// idx / 128 (as the whole-tube kernel's coin words)
+__device__ __forceinline__ uint32_t coin_bit(uint64_t seed, uint64_t idx) {
+  return (coin_word(seed, idx & ~31ULL) >> (idx & 31)) & 1u;
+}
+
+// coins of the 32 indices c .. c+31 (bit k: index c+k), at any alignment
+__device__ __forceinline__ uint32_t coin_run(uint64_t seed, uint64_t c) {
+  const uint32_t sh = (uint32_t)(c & 31);
+  const uint32_t lo = coin_word(seed, c - sh);
+  if (!sh) return lo;
+  return (lo >> sh) | (coin_word(seed, c - sh + 32) << (32 - sh));
+}
+
+// Coin index of site 0 of the instance.
+__device__ __forceinline__ uint64_t coin_base(const Lattice &g, const Inst &in) {
+  return in.coin + (uint64_t)in.s * g.V * g.nt + (uint64_t)in.beta * g.V;
+}
+
+// Dynamic LDS: rows[planes*L] u64 (then the seam-root bitmap) |
+// lab[planes*L*L] u16.
 __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                    Phase ph) {
   extern __shared__ uint64_t lds[];
@@ -1582,33 +1601,83 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *__restrict__ tube
   const Lattice g = a.g;
   const int ns = slab_count(a), slab = wh.local % ns;
   const Inst in = instance_of(a, wh.local / ns, wh.step, ph.call);
-  const int S = g.L * g.L, z0 = slab * a.planes;
-  const int planes = min(a.planes, g.Lz - z0), nsite = planes * S;
+  const int L = g.L, S = L * L, z0 = slab * a.planes;
+  const int planes = min(a.planes, g.Lz - z0), nsite = planes * S, nrows = planes * L;
   const int T = blockDim.x, tid = threadIdx.x;
   uint64_t *rows = lds;
-  uint16_t *lab = reinterpret_cast<uint16_t *>(rows + a.planes * g.L);  // < 65536 sites
+  uint16_t *lab = reinterpret_cast<uint16_t *>(rows + a.planes * L);  // < 65536 sites
   const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
   const size_t ib = instance_index(g, in);
   const int gbase = z0 * S;
-  for (int r = tid; r < planes * g.L; r += T)
-    rows[r] = row_bits(g, z0 * g.L + r, [&](int p, int h) {
+  const auto grow = gv(a.hrow) + ib * (g.V / L) + z0 * L;
+  for (int r = tid; r < nrows; r += T) {
+    const uint64_t q = row_bits(g, z0 * L + r, [&](int p, int h) {
       const size_t o = (size_t)p * g.half + h;
       return gmem(a.spin)[pw.o1 + o] ^ gmem(a.spin)[pw.o2 + o];
     });
-  const uint64_t cidx = in.coin + (uint64_t)in.s * g.V * g.nt + (uint64_t)in.beta * g.V + gbase;
-  for (int c = tid; c < nsite / 32; c += T)
-    gv(a.hcb)[ib * (g.V / 32) + gbase / 32 + c] = coin_word(a.coin_seed, cidx + 32 * c);
+    rows[r] = q;
+    grow[r] = q;
+  }
   __syncthreads();
-  label_rows(g, lab, rows, planes, false);
-  const auto gl = gv(a.hlab) + ib * g.V + gbase;
-  for (int i = tid; i < nsite; i += T) gl[i] = gbase + lab[i];
+  label_rows(g, lab, rows, planes, false);  // ends with a barrier; rows are not read again
+  // the roots of the sites on a seam plane: plane 0 unless it starts an open
+  // tube, plane planes-1 unless it ends one
+  const bool pbc = g.bc == EA_PBC;
+  const bool below = z0 > 0 || pbc, above = z0 + planes < g.Lz || pbc;
+  uint32_t *seam = reinterpret_cast<uint32_t *>(rows);  // nsite bits fit the rows' words
+  for (int w = tid; w < (nsite + 31) / 32; w += T) seam[w] = 0;
+  __syncthreads();
+  for (int i = tid; i < S; i += T) {
+    if (below) {
+      const uint32_t r = lab[i];
+      atomicOr(seam + (r >> 5), 1u << (r & 31));
+    }
+    if (above) {
+      const uint32_t r = lab[(planes - 1) * S + i];
+      atomicOr(seam + (r >> 5), 1u << (r & 31));
+    }
+  }
+  __syncthreads();
+  const auto gl16 = gv(a.hl16) + ib * g.V + gbase;
+  for (int i = tid; i < nsite; i += T) gl16[i] = lab[i];
+  // roots, 32 sites (one coin word) per thread: flip bits of the roots of
+  // clusters inside the slab, seam roots as forest nodes
+  const auto gpar = gv(a.hpar) + ib * g.V + gbase;
+  const int wps = hou_slab_words(g, a.planes);
+  const auto gfl = gv(reinterpret_cast<uint32_t *>(a.hflag)) + (ib * ns + slab) * 4 * wps;
+  const uint64_t cidx = coin_base(g, in) + gbase;
+  for (int w = tid; w < 2 * wps; w += T) {
+    uint32_t fb = 0, nb = 0;
+    if (32 * w < nsite) {
+      const uint32_t coins = coin_run(a.coin_seed, cidx + 32 * w), sb = seam[w];
+      // the group's labels in four 16-byte reads (lanes 64 bytes apart)
+      typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+      const u32x4 *lv = reinterpret_cast<const u32x4 *>(lab + 32 * w);
+      u32x4 v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = lv[q];
+#pragma unroll
+      for (int k = 0; k < 32; ++k) {
+        const int i = 32 * w + k;
+        if (((v[k >> 3][(k >> 1) & 3] >> (16 * (k & 1))) & 0xFFFFu) != (uint32_t)i) continue;
+        if ((sb >> k) & 1u) {
+          nb |= 1u << k;
+          gpar[i] = gbase + i;
+        } else {
+          fb |= coins & (1u << k);
+        }
+      }
+    }
+    gfl[w] = fb;
+    gfl[2 * wps + w] = nb;
+  }
 }
 
 // z bonds between plane z0-1 (or Lz-1 for the periodic wrap) and plane z0,
-// the first plane of the slab, one lane per row y: the two rows' q bits
-// (row_bits) give the overlaps of equal-q runs, and each overlap is united
-// once, at its start, between the two runs' representatives (as in the
-// whole-tube labelling).  The sites of a run already share a root.
+// the first plane of the slab, one lane per row y: the two rows' q bits give
+// the overlaps of equal-q runs, and each overlap is united once, at its
+// start, between the two runs' slab-local roots (every site of a run has
+// its run's root); the same pair is not united twice in a row.
 __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
                                                  const uint64_t *__restrict__ map, Phase ph) {
   using R = Rows<uint64_t>;
@@ -1618,69 +1687,135 @@ __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
   const Lattice g = a.g;
   const int ns = slab_count(a), slab = wh.local % ns;
   const Inst in = instance_of(a, wh.local / ns, wh.step, ph.call);
-  const int L = g.L, y = threadIdx.x, z0 = slab * a.planes;
+  const int L = g.L, S = L * L, y = threadIdx.x, z0 = slab * a.planes;
   if (y >= L) return;  // no barrier follows
   int zl = z0 - 1;
   if (z0 == 0) {
     if (g.bc != EA_PBC) return;
     zl = g.Lz - 1;
   }
-  const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
-  uint32_t *gl = a.hlab + instance_index(g, in) * g.V;  // global unions, no LDS
-  const auto spin = gmem(a.spin);
-  auto qword = [&](int p, int h) {
-    const size_t o = (size_t)p * g.half + h;
-    return spin[pw.o1 + o] ^ spin[pw.o2 + o];
-  };
+  const size_t ib = instance_index(g, in);
+  const auto grow = gv(a.hrow) + ib * (g.V / L);
+  const auto gl16 = gv(a.hl16) + ib * g.V;
+  uint32_t *par = a.hpar + ib * g.V;  // global unions, no LDS
   const uint64_t full = L == 64 ? ~0ULL : ((1ULL << L) - 1);
-  const uint64_t qu = row_bits(g, z0 * L + y, qword), qd = row_bits(g, zl * L + y, qword);
-  const uint64_t bu = R::bounds(qu, L, full), bd = R::bounds(qd, L, full);
+  const uint64_t qu = grow[z0 * L + y], qd = grow[zl * L + y];
   const uint64_t lu = ~(qu ^ (qu << 1)) & full & ~1ULL;  // q[x] == q[x-1]
   const uint64_t ld = ~(qd ^ (qd << 1)) & full & ~1ULL;
   uint64_t todo = ~(qu ^ qd) & full & ~(lu & ld);
+  const uint32_t bu = (uint32_t)z0 * S, bd = (uint32_t)(zl / a.planes) * a.planes * S;
   const uint32_t ru = (uint32_t)(z0 * L + y) * L, rd = (uint32_t)(zl * L + y) * L;
+  uint32_t pu = ~0u, pd = ~0u;
   while (todo) {
     const int x = R::lo(todo);
     todo &= todo - 1;
-    unite(gl, ru + R::rep(bu, x), rd + R::rep(bd, x));
+    const uint32_t nu = bu + gl16[ru + x], nd = bd + gl16[rd + x];
+    if (nu == pu && nd == pd) continue;
+    pu = nu;
+    pd = nd;
+    unite(par, nu, nd);
   }
 }
 
-__device__ __forceinline__ int word_blocks(const Lattice &g) { return (2 * g.n + 255) / 256; }
-
-// The move: each of a word's 64 sites finds its cluster root (a slab's
-// label is the slab-local root; the seams linked those roots, so the chain
-// is short), and the root's coin decides.  With labels kept, the roots are
-// stored too.
+// Dynamic LDS: flip[words] u64 | move[words] u64 | mrow[planes*L] u64 |
+// seam roots[2*L*L] u16.
 __global__ void __launch_bounds__(256) k_hou_apply(const Tube *__restrict__ tubes,
                                                    const uint64_t *__restrict__ map, Phase ph) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  extern __shared__ uint64_t lds[];
+  __shared__ int nseam;
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
-  const int nb = word_blocks(g), bxi = wh.local % nb;
-  const Inst in = instance_of(a, wh.local / nb, wh.step, ph.call);
-  const int w = bxi * blockDim.x + threadIdx.x;
-  if (w >= 2 * g.n) return;
+  const int ns = slab_count(a), slab = wh.local % ns;
+  const Inst in = instance_of(a, wh.local / ns, wh.step, ph.call);
+  const int L = g.L, S = L * L, z0 = slab * a.planes;
+  const int planes = min(a.planes, g.Lz - z0), nsite = planes * S, nrows = planes * L;
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int wps = hou_slab_words(g, a.planes);
+  uint64_t *flip = lds, *move = lds + wps, *mrow = move + wps;
+  uint16_t *seam = reinterpret_cast<uint16_t *>(mrow + a.planes * L);
   const size_t ib = instance_index(g, in);
-  const uint32_t *gl = a.hlab + ib * g.V;
-  const auto cb = gv(a.hcb) + ib * (g.V / 32);
-  const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
-  const int p = w >= g.n, h = w - p * g.n;
-  const size_t o = (size_t)p * g.half + h;
-  const int base = site_base(g, p, h);
-  uint64_t m = 0;
-  for (int b = 0; b < 64; ++b) {
-    const int site = base + copy_offset(g, b);
-    const uint32_t root = find_root(gl, site);
-    m |= (uint64_t)((cb[root >> 5] >> (root & 31)) & 1u) << b;
-    if (a.keep) gv(a.keep)[ib * g.V + site] = root;
+  const uint32_t gbase = (uint32_t)z0 * S;
+  const auto gfl = gv(a.hflag) + (ib * ns + slab) * 2 * wps;
+  const auto gl16 = gv(a.hl16) + ib * g.V + gbase;
+  const uint32_t *par = a.hpar + ib * g.V;
+  const uint64_t cidx = coin_base(g, in);
+  if (tid == 0) nseam = 0;
+  for (int w = tid; w < wps; w += T) flip[w] = gfl[w];
+  __syncthreads();
+  for (int w = tid; w < wps; w += T) {  // list the seam roots (at most the 2 S seam sites)
+    uint64_t nb = gfl[wps + w];
+    if (!nb) continue;
+    int j = atomicAdd(&nseam, __popcll(nb));
+    while (nb) {
+      seam[j++] = (uint16_t)(64 * w + __ffsll((unsigned long long)nb) - 1);
+      nb &= nb - 1;
+    }
   }
-  const auto spin = gmem(a.spin);
-  const uint64_t x1 = spin[pw.o1 + o], x2 = spin[pw.o2 + o];
-  const uint64_t qn = x1 ^ x2 ^ m;
-  spin[pw.o1 + o] = x1 ^ qn;
-  spin[pw.o2 + o] = x2 ^ qn;
+  __syncthreads();
+  // each seam root takes the coin of its root in the forest
+  for (int j = tid; j < nseam; j += T) {
+    const uint32_t l = seam[j];
+    const uint32_t root = find_root(par, gbase + l);
+    if (coin_bit(a.coin_seed, cidx + root))
+      atomicOr(reinterpret_cast<unsigned long long *>(flip + (l >> 6)), 1ULL << (l & 63));
+  }
+  __syncthreads();
+  // move bit of every site from its root's flip bit, 8 sites per lane
+  uint8_t *mb = reinterpret_cast<uint8_t *>(move);
+  const auto lab8 = reinterpret_cast<const global_t<u32x4> *>(gl16);
+  for (int c = tid; 8 * c < nsite; c += T) {
+    const u32x4 v = lab8[c];
+    uint32_t byte = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t l = (v[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+      byte |= (uint32_t)((flip[l >> 6] >> (l & 63)) & 1ULL) << k;
+    }
+    mb[c] = (uint8_t)byte;
+  }
+  __syncthreads();
+  const uint64_t full = L == 64 ? ~0ULL : ((1ULL << L) - 1);
+  for (int r = tid; r < nrows; r += T) {  // row r: move bits r L .. r L + L - 1
+    const int b0 = r * L, w0 = b0 >> 6, sh = b0 & 63;
+    uint64_t m = move[w0] >> sh;
+    if (sh + L > 64) m |= move[w0 + 1] << (64 - sh);
+    mrow[r] = m & full;
+  }
+  __syncthreads();
+  // pieces (p, h, bz) of the slab's planes: plane z holds the words of
+  // gz = z mod MLz, piece bz = z / MLz, bits bx + 4 by
+  const auto s16 = gv(reinterpret_cast<uint16_t *>(a.spin));
+  const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
+  const int pp = 2 * g.hs;  // pieces per plane
+  for (int t = tid; t < planes * pp; t += T) {
+    const int zl = t / pp, rem = t - zl * pp, p = rem >= g.hs, hh = rem - p * g.hs;
+    const int z = z0 + zl, bz = qdiv(z, g.mMLz), gz = z - bz * g.MLz;
+    const int yb = qdiv(hh, g.mhl), ax = 2 * (hh - yb * g.hl) + ((p + yb + gz) & 1);
+    uint32_t m = 0;
+#pragma unroll
+    for (int by = 0; by < 4; ++by) {
+      const uint64_t row = mrow[by * g.ML + yb + L * zl] >> ax;
+#pragma unroll
+      for (int bx = 0; bx < 4; ++bx) m |= (uint32_t)((row >> (bx * g.ML)) & 1ULL) << (bx + 4 * by);
+    }
+    const size_t o = (size_t)p * g.half + (size_t)gz * g.hs + hh;
+    const size_t k1 = (pw.o1 + o) * 4 + bz, k2 = (pw.o2 + o) * 4 + bz;
+    const uint32_t x1 = s16[k1], x2 = s16[k2];
+    const uint32_t qn = x1 ^ x2 ^ m;  // q' = q ^ coin(label)
+    s16[k1] = (uint16_t)(x1 ^ qn);
+    s16[k2] = (uint16_t)(x2 ^ qn);
+  }
+  if (a.keep) {  // the label of every site: its cluster's smallest site index
+    const auto out = gv(a.keep) + ib * g.V + gbase;
+    for (int i = tid; i < nsite; i += T) {
+      const uint32_t l = gl16[i];
+      const bool node = (gfl[wps + (l >> 6)] >> (l & 63)) & 1ULL;
+      out[i] = node ? find_root(par, gbase + l) : gbase + l;
+    }
+  }
 }
 
 inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -1723,6 +1858,24 @@ size_t hou_slab_lds(const Lattice &g, int planes) {
   return 8 * (size_t)planes * g.L + 2 * (size_t)planes * g.L * g.L;
 }
 
+size_t hou_apply_lds(const Lattice &g, int planes) {
+  return 8 * (2 * (size_t)hou_slab_words(g, planes) + (size_t)planes * g.L) +
+         2 * 2 * (size_t)g.L * g.L;
+}
+
+int hou_slab_planes(const Lattice &g, size_t lds_slab) {
+  int planes = std::max(1, std::min(g.Lz, kHouSlabSites / (g.L * g.L)));
+  while (planes > 1 && hou_slab_lds(g, planes) > lds_slab) --planes;
+  return planes;
+}
+
+size_t hou_slab_bytes(const Lattice &g, int planes) {
+  const size_t inst = (size_t)2 * g.ns * g.ctm;
+  const size_t slabs = (g.Lz + planes - 1) / planes;
+  return inst * (4 * (size_t)g.V + 2 * (size_t)g.V + 8 * (size_t)(g.V / g.L) +
+                 16 * slabs * hou_slab_words(g, planes));
+}
+
 Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
   Plan p;
   p.ntubes = (int)tubes.size();
@@ -1759,8 +1912,9 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
       const long long slabs = (g.Lz + t.planes - 1) / t.planes;
       add(kMapSlab, slabs * inst);
       add(kMapSeam, slabs * inst);
-      add(kMapApply, (long long)blocks_for(2LL * g.n, 256) * inst);
+      add(kMapApply, slabs * inst);
       p.lds_slab = std::max(p.lds_slab, hou_slab_lds(g, t.planes));
+      p.lds_apply = std::max(p.lds_apply, hou_apply_lds(g, t.planes));
     }
   }
   return p;
@@ -1778,6 +1932,7 @@ hipError_t upload_plan(const Plan &p, DevPlan *d) {
   d->lds32 = p.lds32;
   d->lds64 = p.lds64;
   d->lds_slab = p.lds_slab;
+  d->lds_apply = p.lds_apply;
   d->tile_words = p.tile_words;
   d->shape = p.shape;
   d->identity = p.ntubes == 1;
@@ -1924,8 +2079,9 @@ hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStr
                        tubes, p.map(kMapSlab), ph);
     hipLaunchKernelGGL(k_hou_seam, dim3((unsigned)p.cnt[kMapSeam]), dim3(64), 0, s, tubes,
                        p.map(kMapSeam), ph);
-    hipLaunchKernelGGL(k_hou_apply, dim3((unsigned)p.cnt[kMapApply]), dim3(256), 0, s, tubes,
-                       p.map(kMapApply), ph);
+    allow_lds(reinterpret_cast<const void *>(k_hou_apply), p.lds_apply);
+    hipLaunchKernelGGL(k_hou_apply, dim3((unsigned)p.cnt[kMapApply]), dim3(256), p.lds_apply, s,
+                       tubes, p.map(kMapApply), ph);
   }
   return hipGetLastError();
 }

@@ -73,7 +73,13 @@ struct Tube {
   const int32_t *toff, *tlen;
   double *esum;             // [ns][nr][nt] by beta
   uint8_t *hist;            // [rows][ns][nr][nt] beta of clone
-  uint32_t *hlab, *hcb;     // slab Houdayer: labels, root coins
+  // slab Houdayer, per instance: parents of the forest of the roots that
+  // reach a seam [V], slab-local root of every site [V], q bits of every row
+  // [V/L], and per slab the flip bits of the roots whose cluster stays inside
+  // it followed by the bits of the roots that reach a seam [slabs][2][words]
+  uint32_t *hpar;
+  uint16_t *hl16;
+  uint64_t *hrow, *hflag;
   uint32_t *keep;           // copy of the last labels, or NULL
   int planes;               // slab Houdayer planes; 0: whole tube in one block
   int nbx;                  // sweep blocks per beta
@@ -136,7 +142,7 @@ constexpr int kSweepPosMin = 32;
 struct Plan {
   std::vector<uint64_t> map[kMapKinds];
   Shape shape;
-  size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0;
+  size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0;
   int tile_words = 0;  // tile words per thread of a tiled sweep (the widest tube's)
   int ntubes = 0;
 };
@@ -148,7 +154,7 @@ struct DevPlan {
   uint64_t *buf = nullptr;
   size_t off[kMapKinds] = {};
   size_t cnt[kMapKinds] = {};
-  size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0;
+  size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0;
   int tile_words = 0;  // tile words per thread of a tiled sweep (the widest tube's)
   Shape shape;
   bool identity = false;
@@ -190,7 +196,16 @@ hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
 // otherwise in slabs of about kHouSlabSites sites.  Both need L <= 64.
 size_t hou_whole_lds(const Lattice &g);
 size_t hou_slab_lds(const Lattice &g, int planes);
+size_t hou_apply_lds(const Lattice &g, int planes);
 constexpr int kHouSlabSites = 65536;  // slab-local labels are 16-bit
+// Planes per slab: the thickest slab whose labelling fits lds_slab bytes.
+int hou_slab_planes(const Lattice &g, size_t lds_slab);
+// 64-bit words of one slab's flip (or seam-root) bits.
+inline __host__ __device__ int hou_slab_words(const Lattice &g, int planes) {
+  return (planes * g.L * g.L + 63) / 64;
+}
+// Device bytes of the slab Houdayer buffers of a tube.
+size_t hou_slab_bytes(const Lattice &g, int planes);
 constexpr int kHouMaxL = 64;
 
 }  // namespace eatube

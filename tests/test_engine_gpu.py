@@ -239,9 +239,9 @@ def test_full_size_houdayer_involution_and_conservation(bc):
 
 
 def test_full_size_houdayer_31_samples():
-    """L=48 Lz=384 with all 31 samples: 992 cluster instances, so the
-    label-flatten launch has 3456 x 992 > 2^20 workgroups.  Labels of the
-    last instances equal the oracle's, and the move is an involution."""
+    """L=48 Lz=384 with all 31 samples: 992 cluster instances, each
+    labelled in 24 slabs joined at their seams.  Labels of the last
+    instances equal the oracle's, and the move is an involution."""
     e = full_size_sim(bc=1, ns=31)
     eng = e.engine
     e.steps(1, cluster=False)
