@@ -62,10 +62,8 @@ int dalloc(T **p, size_t count) {
 struct StepShape {
   int mesfr = 0, houfr = 0, ptfr = 0;
   bool cluster = false;   // some tube has cluster moves
-  bool unfused = false;   // some tube measures with a separate energy pass
   bool operator==(const StepShape &o) const {
-    return mesfr == o.mesfr && houfr == o.houfr && ptfr == o.ptfr && cluster == o.cluster &&
-           unfused == o.unfused;
+    return mesfr == o.mesfr && houfr == o.houfr && ptfr == o.ptfr && cluster == o.cluster;
   }
 };
 
@@ -568,9 +566,10 @@ struct Target {
 };
 
 // One step of main()'s `it` loop (main.cu:149-228) for every tube of the
-// table: mesfr x { houfr x { ptfr sweeps, PT }, Houdayer }, then the energy
-// measure of tubes that do not measure inside PT.  Every PT follows a
-// sweep, so PT takes its energies from the sweep's partials.  With step >= 0
+// table: mesfr x { houfr x { ptfr sweeps, PT }, Houdayer }.  Every PT
+// follows a sweep, so PT takes its energies from the sweep's partials; the
+// measuring PT adds them to the energy sums and history, and a cluster move
+// after it corrects the sums of the chains it changed.  With step >= 0
 // (one tube launched step by step) the launches carry the step index;
 // with step < 0 (a captured step, replayed) they read the tubes' step
 // counters, which the step's last launch advances.
@@ -595,14 +594,6 @@ static int enqueue_step(const Target &x, const StepShape &sh, int step = -1) {
       HIP_TRY(launch_houdayer(x.dt, *x.plan, ph, x.stream));
     }
   }
-  if (sh.unfused) {
-    {
-      Timed tm(x.timer, EA_K_ENERGY);
-      HIP_TRY(launch_energy(x.dt, *x.plan, true, step, x.stream));
-    }
-    Timed tm(x.timer, EA_K_MEASURE);
-    HIP_TRY(launch_measure(x.dt, *x.plan, true, step, x.stream));
-  }
   if (step < 0) HIP_TRY(launch_step_end(x.dt, x.ntubes, x.stream));
   return EA_OK;
 }
@@ -617,7 +608,7 @@ static int ensure_energy(ea_engine *e) {
     HIP_TRY(launch_resolve(e->dt, e->plan, 0, e->stream));
   } else if (!e->ener_valid) {
     Timed t(e, EA_K_ENERGY);
-    HIP_TRY(launch_energy(e->dt, e->plan, false, 0, e->stream));
+    HIP_TRY(launch_energy(e->dt, e->plan, 0, e->stream));
   }
   e->part_pending = false;
   e->ener_valid = true;
@@ -646,7 +637,7 @@ extern "C" int ea_energy(ea_engine *e, int32_t *out) {
   RC_TRY(begin_calls(e, 1, 0, 0, 0, 1));
   {
     Timed t(e, EA_K_ENERGY);
-    HIP_TRY(launch_energy(e->dt, e->plan, false, 0, e->stream));
+    HIP_TRY(launch_energy(e->dt, e->plan, 0, e->stream));
   }
   e->part_pending = false;
   e->ener_valid = true;
@@ -747,12 +738,11 @@ extern "C" int ea_houdayer(ea_engine *e, const int32_t pairs[4], uint64_t coin_s
   e->t.coin0 = *rng_offset;
   e->t.coin_per_call = coin_per_call(e->g());
   RC_TRY(begin_calls(e, 1, 0, 1, 0, 1));
-  if (e->t.planes == 0) RC_TRY(ensure_energy(e));  // the whole-tube kernel updates ener
+  RC_TRY(ensure_energy(e));  // the move updates the two changed chains' energies
   {
     Timed t(e, EA_K_HOUDAYER);
     HIP_TRY(launch_houdayer(e->dt, e->plan, Phase{0, 0, 0, 0, 0}, e->stream));
   }
-  if (e->t.planes > 0) e->part_pending = e->ener_valid = false;
   *rng_offset += e->t.coin_per_call;
   return EA_OK;
 }
@@ -825,7 +815,6 @@ extern "C" int ea_run(ea_engine *e, const ea_schedule *sch, uint64_t *rng_offset
   sh.houfr = sch->houfr;
   sh.ptfr = sch->ptfr;
   sh.cluster = sch->cluster != 0;
-  sh.unfused = !measured_in_pt(e->t);
   const Target x = engine_target(e);
   for (int it = 0; it < sch->n_it; ++it) RC_TRY(enqueue_step(x, sh, it));
   HIP_TRY(launch_set_step(e->t.step, sch->n_it, e->stream));
@@ -947,7 +936,6 @@ extern "C" int ea_batch_run(ea_batch *b, const ea_schedule *sch, uint64_t *rng_o
     tubes[i].nbx = sweep_partials_x(e->g(), b->launch);
     cluster[i] = e->t.cluster;
     sh.cluster = sh.cluster || e->t.cluster;
-    sh.unfused = sh.unfused || !measured_in_pt(e->t);
   }
   HIP_TRY(hipStreamSynchronize(b->stream));
   HIP_TRY(hipMemcpy(b->dt, tubes.data(), sizeof(Tube) * n, hipMemcpyHostToDevice));

@@ -887,7 +887,7 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
     satt[i] = att[i];
     sraw[i] = raw[i];
   }
-  const bool measure = ph.measure && measured_in_pt(a);
+  const bool measure = ph.measure;
   if (measure)
     for (int b = lane; b < nt; b += 64) sesum[b] = esum[b];
   if (ph.energy) {
@@ -1037,24 +1037,7 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
   }
 }
 
-// --------------------------------------------------------------- measures
-
-// Per-beta energy sums and the beta-of-clone history (main.cu:208-228).
-__global__ void __launch_bounds__(64) k_measure(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
-                                                int step) {
-  const Where wh = locate(tubes, map, step);
-  const Tube &a = *wh.t;
-  if (wh.step >= a.nsteps) return;
-  const Lattice g = a.g;
-  const int chain = wh.local, nt = g.nt;
-  const int itb = a.it0 + wh.step;
-  const int row = itb % a.frec == a.frec - 1 ? itb / a.frec : -1;
-  for (int b = threadIdx.x; b < nt; b += blockDim.x) {
-    const int c = a.wctb[(size_t)chain * nt + b];
-    gv(a.esum)[(size_t)chain * nt + b] += (double)gv(a.ener)[(size_t)chain * nt + c];
-    if (row >= 0) gv(a.hist)[((size_t)row * g.ns * g.nr + chain) * nt + c] = (uint8_t)b;
-  }
-}
+// ------------------------------------------------------------ step counter
 
 // Advances the step counter of every tube that has steps left.
 __global__ void k_step_end(const Tube *__restrict__ tubes, int ntubes) {
@@ -1818,6 +1801,56 @@ __global__ void __launch_bounds__(256) k_hou_apply(const Tube *__restrict__ tube
   }
 }
 
+// Energies of the two configurations a slab move changed, as the
+// whole-tube kernel recomputes them (the Measure_Energy after a cluster
+// move, main.cu:209): one workgroup per instance over the even words of the
+// two replicas; ener of the two chains is replaced and, when the step
+// measures, their energy sums move by the difference.
+__global__ void __launch_bounds__(512) k_hou_energy(const Tube *__restrict__ tubes,
+                                                    const uint64_t *__restrict__ map, Phase ph) {
+  __shared__ int esum[2];
+  const Where wh = locate(tubes, map, ph.step);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const Inst in = instance_of(a, wh.local, wh.step, ph.call);
+  const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
+  const int T = blockDim.x, tid = threadIdx.x;
+  if (tid < 2) esum[tid] = 0;
+  __syncthreads();
+  int e0 = 0, e1 = 0;
+  for (int t = tid; t < 2 * g.n; t += T) {
+    const int rep = t >= g.n, h = t - rep * g.n;
+    const auto base = gmem(a.spin) + (rep ? pw.o2 : pw.o1);
+    const Nbrs nb = neighbours(g, 0, h);
+    uint64_t sc[6], Jw[6];
+    load_neighbours(g, base + g.half, 0, nb, 0, sc, 1);
+    load_couplings(g, a.J, in.s, 0, h, Jw);
+    const uint64_t sv = base[h];
+    int u = 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
+    (rep ? e1 : e0) += 6 * 64 - 2 * u;
+  }
+  e0 = wave_sum(e0);
+  e1 = wave_sum(e1);
+  if ((tid & 63) == 0) {
+    if (e0) atomicAdd(esum, e0);
+    if (e1) atomicAdd(esum + 1, e1);
+  }
+  __syncthreads();
+  if (tid < 2) {
+    const int r = gv(in.pairs)[2 * in.grp + tid];
+    const size_t chain = (size_t)in.s * g.nr + r;
+    const int c = gv(a.wctb)[chain * g.nt + in.beta];
+    const int old = gv(a.ener)[chain * g.nt + c];
+    gv(a.ener)[chain * g.nt + c] = esum[tid];
+    // the sums already hold the pre-move energy; all values are integers
+    // below 2^53, so adding the difference is exact
+    if (ph.measure) gv(a.esum)[chain * g.nt + in.beta] += (double)(esum[tid] - old);
+  }
+}
+
 inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -1892,10 +1925,6 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
     add(kMapEnergy, (long long)energy_blocks_host(g) * g.nt * g.ns);
     add(kMapChain, (long long)g.ns * g.nr);
     if (!t.cluster || g.ctm <= 0) continue;
-    if (!measured_in_pt(t)) {
-      add(kMapEnergyU, (long long)energy_blocks_host(g) * g.nt * g.ns);
-      add(kMapChainU, (long long)g.ns * g.nr);
-    }
     const long long inst = 2LL * g.ns * g.ctm;
     if (t.planes == 0) {
       if (hou_idx16(g)) {
@@ -1913,6 +1942,7 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
       add(kMapSlab, slabs * inst);
       add(kMapSeam, slabs * inst);
       add(kMapApply, slabs * inst);
+      add(kMapHouEnergy, inst);
       p.lds_slab = std::max(p.lds_slab, hou_slab_lds(g, t.planes));
       p.lds_apply = std::max(p.lds_apply, hou_apply_lds(g, t.planes));
     }
@@ -2006,15 +2036,13 @@ hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_energy(const Tube *tubes, const DevPlan &p, bool unfused_only, int step,
-                         hipStream_t s) {
-  const int me = unfused_only ? kMapEnergyU : kMapEnergy, mc = unfused_only ? kMapChainU : kMapChain;
-  if (p.cnt[me])
-    hipLaunchKernelGGL(k_energy, dim3((unsigned)p.cnt[me]), dim3(256), 0, s, tubes, p.map(me),
-                       step);
-  if (p.cnt[mc])
-    hipLaunchKernelGGL(k_energy_sum<false>, dim3((unsigned)p.cnt[mc]), dim3(64), 0, s, tubes,
-                       p.map(mc), step);
+hipError_t launch_energy(const Tube *tubes, const DevPlan &p, int step, hipStream_t s) {
+  if (p.cnt[kMapEnergy])
+    hipLaunchKernelGGL(k_energy, dim3((unsigned)p.cnt[kMapEnergy]), dim3(256), 0, s, tubes,
+                       p.map(kMapEnergy), step);
+  if (p.cnt[kMapChain])
+    hipLaunchKernelGGL(k_energy_sum<false>, dim3((unsigned)p.cnt[kMapChain]), dim3(64), 0, s,
+                       tubes, p.map(kMapChain), step);
   return hipGetLastError();
 }
 
@@ -2029,15 +2057,6 @@ hipError_t launch_pt(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t 
   if (p.cnt[kMapChain])
     hipLaunchKernelGGL(k_pt, dim3((unsigned)p.cnt[kMapChain]), dim3(64), 0, s, tubes,
                        p.map(kMapChain), ph);
-  return hipGetLastError();
-}
-
-hipError_t launch_measure(const Tube *tubes, const DevPlan &p, bool unfused_only, int step,
-                          hipStream_t s) {
-  const int mc = unfused_only ? kMapChainU : kMapChain;
-  if (p.cnt[mc])
-    hipLaunchKernelGGL(k_measure, dim3((unsigned)p.cnt[mc]), dim3(64), 0, s, tubes, p.map(mc),
-                       step);
   return hipGetLastError();
 }
 
@@ -2082,6 +2101,8 @@ hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStr
     allow_lds(reinterpret_cast<const void *>(k_hou_apply), p.lds_apply);
     hipLaunchKernelGGL(k_hou_apply, dim3((unsigned)p.cnt[kMapApply]), dim3(256), p.lds_apply, s,
                        tubes, p.map(kMapApply), ph);
+    hipLaunchKernelGGL(k_hou_energy, dim3((unsigned)p.cnt[kMapHouEnergy]), dim3(512), 0, s,
+                       tubes, p.map(kMapHouEnergy), ph);
   }
   return hipGetLastError();
 }

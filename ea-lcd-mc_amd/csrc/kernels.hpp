@@ -106,19 +106,13 @@ struct Phase {
                    // step by step); -1: read the tube's step counter
 };
 
-// Block maps of every kernel kind over one table of tubes.
-// kMapEnergyU / kMapChainU cover only the tubes whose step measures with a
-// separate energy pass (cluster moves on the slab path).
+// Block maps of every kernel kind over one table of tubes (kMapHouEnergy:
+// one block per cluster instance of the tubes labelled in slabs).
 enum MapKind {
   kMapSweep, kMapEnergy, kMapChain, kMapWhole16, kMapWhole32, kMapWhole64, kMapSlab,
-  kMapSeam, kMapApply, kMapEnergyU, kMapChainU, kMapKinds
+  kMapSeam, kMapApply, kMapHouEnergy, kMapKinds
 };
 
-// A tube's step measures its energies inside PT (and the whole-tube cluster
-// move corrects them) unless its cluster moves take the slab path.
-inline __host__ __device__ bool measured_in_pt(const Tube &t) {
-  return !t.cluster || t.planes == 0;
-}
 // Launch shape of a table's kernels (chosen by measurement, ea_set_shape /
 // ea_batch_set_shape).  Sweep: kind 0 shares each position's random words
 // among the samples through LDS (1024-thread blocks of sweep_positions = 32
@@ -179,13 +173,10 @@ void free_plan(DevPlan *d);
 
 // Launches over a device tube table.  Each returns hipGetLastError().
 hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
-hipError_t launch_energy(const Tube *tubes, const DevPlan &p, bool unfused_only, int step,
-                         hipStream_t s);
+hipError_t launch_energy(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_pt(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
 hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
-hipError_t launch_measure(const Tube *tubes, const DevPlan &p, bool unfused_only, int step,
-                          hipStream_t s);
 hipError_t launch_step_end(const Tube *tubes, int ntubes, hipStream_t s);
 hipError_t launch_set_step(int *step, int value, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);

@@ -206,7 +206,8 @@ enum {
   EA_K_ENERGY = 1,   /* standalone energy (incl. its zero fill) */
   EA_K_PT = 2,       /* parallel tempering */
   EA_K_HOUDAYER = 3, /* all passes of one cluster move */
-  EA_K_MEASURE = 4,  /* energy sums and history */
+  EA_K_MEASURE = 4,  /* energy sums and history (made inside PT and the
+                        cluster move: no launches of their own) */
   EA_K_KINDS = 5
 };
 int ea_set_timing(ea_engine *e, int on);
