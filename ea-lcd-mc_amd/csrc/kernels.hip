@@ -583,8 +583,9 @@ __device__ __forceinline__ void lds_barrier() {
 // One half-sweep, kind 2 (tiled): blocks as kind 1 (256 consecutive half
 // sites of one replica at one beta), but a sample's six neighbour words come
 // from an LDS tile of the opposite parity's clone row over the block's run
-// widened by hs on each side (every x, y and unwrapped z neighbour): each
-// thread loads K <= 3 tile words per sample instead of six neighbours.
+// widened by hs on each side (every x, y and unwrapped z neighbour), read
+// as 16-byte chunks from the even half site at or below its start: each
+// thread loads KC <= 2 chunks per sample instead of six neighbours.
 // Samples go in pairs; the next pair's tiles are loaded with this pair's
 // words and staged after the update (double buffer, barriers that leave
 // global traffic in flight).  All loads are unconditional (clamped
@@ -597,15 +598,20 @@ __host__ __device__ inline int tile_width(const Lattice &g) { return kLaneThread
 inline int tile_words(const Lattice &g) {
   return (tile_width(g) + kLaneThreads - 1) / kLaneThreads;
 }
+// 16-byte chunks per thread: the tile from the even half site at or below
+// its start covers at most tile_width + 1 half sites.
+inline int tile_chunks(const Lattice &g) {
+  return (tile_width(g) + 1 + 2 * kLaneThreads - 1) / (2 * kLaneThreads);
+}
 
-template <int K, bool XCD>
+template <int KC, bool XCD>
 __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restrict__ tubes,
                                                              const uint64_t *__restrict__ map,
                                                              Phase ph) {
   __shared__ __align__(16) uint32_t lut[EA_LUT_BYTES / 4];
   __shared__ uint32_t clone_off[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
   __shared__ uint32_t ctr[EA_MAX_REPLICAS][5];
-  __shared__ uint64_t tile[2][2][K * kLaneThreads];  // [buffer][sample of the pair][x]
+  __shared__ __align__(16) uint64_t tile[2][2][2 * KC * kLaneThreads];  // [buffer][sample][x]
   const Where wh = locate_block(tubes, map, ph.step,
                                 XCD ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x);
   const Tube &a = *wh.t;
@@ -629,21 +635,24 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
   const int p = ph.half & 1;
   const LaneSite ls = lane_site(g, p, h);
   const uint32_t opp = (uint32_t)((1 - p) * g.half);
-  const int W = tile_width(g), lo = h0 - g.hs;  // the tile holds half sites [lo, lo + W)
+  // the neighbours lie in half sites [lo, lo + W); slot x of the tile holds
+  // half site lo2 + x, lo2 even, so that slot pairs are 16-byte chunks
+  const int W = tile_width(g), lo = h0 - g.hs, lo2 = lo & ~1;
+  const int nchunk = (lo + W - lo2 + 1) >> 1;
   int ti[6];
 #pragma unroll
   for (int d = 0; d < 6; ++d) {
     const int x = (int)(ls.nb[d] - opp) - lo;
-    ti[d] = x >= 0 && x < W ? x : -1;  // -1: a z neighbour across the tube's wrap
+    ti[d] = x >= 0 && x < W ? x + (lo - lo2) : -1;  // -1: a z neighbour across the wrap
   }
-  // this thread's tile slots (clamped: a slot past the tile or the tube
-  // repeats a word nobody reads)
-  int tx_[K];
-  uint32_t th_[K];
+  // this thread's chunks (clamped: a chunk past the tile or the tube repeats
+  // words nobody reads; n is even, so no chunk straddles the row's ends)
+  int tc_[KC];
+  uint32_t th_[KC];
 #pragma unroll
-  for (int i = 0; i < K; ++i) {
-    tx_[i] = min(tx + i * kLaneThreads, W - 1);
-    th_[i] = (uint32_t)min(max(lo + tx_[i], 0), g.n - 1);
+  for (int i = 0; i < KC; ++i) {
+    tc_[i] = min(tx + i * kLaneThreads, nchunk - 1);
+    th_[i] = (uint32_t)min(max(lo2 + 2 * tc_[i], 0), g.n - 2);
   }
   uint64_t *const spin = a.spin;
   const uint64_t *const J = a.J;
@@ -655,17 +664,20 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
   const size_t sstride = 2 * (size_t)g.half, jstride = 12 * (size_t)g.n;
   const uint32_t jpair = 2 * (uint32_t)g.n;
 
-  uint64_t tv[2][K];
+  // row offsets are even (n is), so a chunk is 16-byte aligned
+  typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+  u64x2 tv[2][KC];
   auto fetch = [&](int slot, int s) {
     const gword *S = gmem(spin + s * sstride) + clone_off[s * g.nr + r] + opp;
+    const auto S2 = reinterpret_cast<const global_t<u64x2> *>(S);
 #pragma unroll
-    for (int i = 0; i < K; ++i) tv[slot][i] = S[th_[i]];
+    for (int i = 0; i < KC; ++i) tv[slot][i] = S2[th_[i] >> 1];
   };
   auto stage = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-      tile[buf][0][tx_[i]] = tv[0][i];
-      tile[buf][1][tx_[i]] = tv[1][i];
+    for (int i = 0; i < KC; ++i) {
+      *reinterpret_cast<u64x2 *>(&tile[buf][0][2 * tc_[i]]) = tv[0][i];
+      *reinterpret_cast<u64x2 *>(&tile[buf][1][2 * tc_[i]]) = tv[1][i];
     }
   };
 
@@ -1921,7 +1933,7 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
       for (long long i = 0; i < count; ++i) p.map[kind].push_back(tag | (uint64_t)i);
     };
     add(kMapSweep, sweep_grid(t, shape));
-    if (shape.sweep_kind == 2) p.tile_words = std::max(p.tile_words, tile_words(g));
+    if (shape.sweep_kind == 2) p.tile_words = std::max(p.tile_words, tile_chunks(g));
     add(kMapEnergy, (long long)energy_blocks_host(g) * g.nt * g.ns);
     add(kMapChain, (long long)g.ns * g.nr);
     if (!t.cluster || g.ctm <= 0) continue;
@@ -2011,11 +2023,8 @@ hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream
     if (p.tile_words <= 1)
       hipLaunchKernelGGL((sh.sweep_xcd ? k_sweep_tile<1, true> : k_sweep_tile<1, false>), grid,
                          blk, 0, s, tubes, m, ph);
-    else if (p.tile_words == 2)
-      hipLaunchKernelGGL((sh.sweep_xcd ? k_sweep_tile<2, true> : k_sweep_tile<2, false>), grid,
-                         blk, 0, s, tubes, m, ph);
     else
-      hipLaunchKernelGGL((sh.sweep_xcd ? k_sweep_tile<3, true> : k_sweep_tile<3, false>), grid,
+      hipLaunchKernelGGL((sh.sweep_xcd ? k_sweep_tile<2, true> : k_sweep_tile<2, false>), grid,
                          blk, 0, s, tubes, m, ph);
   } else if (sh.sweep_kind == 1) {
     if (sh.sweep_xcd)

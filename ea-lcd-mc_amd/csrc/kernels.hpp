@@ -137,7 +137,7 @@ struct Plan {
   std::vector<uint64_t> map[kMapKinds];
   Shape shape;
   size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0;
-  int tile_words = 0;  // tile words per thread of a tiled sweep (the widest tube's)
+  int tile_words = 0;  // 16-byte tile chunks per thread of a tiled sweep (the widest tube's)
   int ntubes = 0;
 };
 
@@ -149,7 +149,7 @@ struct DevPlan {
   size_t off[kMapKinds] = {};
   size_t cnt[kMapKinds] = {};
   size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0;
-  int tile_words = 0;  // tile words per thread of a tiled sweep (the widest tube's)
+  int tile_words = 0;  // 16-byte tile chunks per thread of a tiled sweep (the widest tube's)
   Shape shape;
   bool identity = false;
   const uint64_t *map(int k) const { return identity ? nullptr : buf + off[k]; }
