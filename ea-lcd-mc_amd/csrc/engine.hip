@@ -717,7 +717,8 @@ extern "C" int ea_set_shape(ea_engine *e, const ea_shape *shape) {
   if (!e) return fail(EA_E_ARG, "NULL engine");
   RC_TRY(check_shape(shape));
   if (!shape_fits(e->g(), to_shape(shape)))
-    return fail(EA_E_ARG, "sweep_kind 2: this tube's tile (256 + L*L/16 half sites) exceeds 768");
+    return fail(EA_E_ARG, "sweep_kind %d: this tube's tile (256 + L*L/16 half sites) exceeds 768",
+                shape->sweep_kind);
   if (e->part_pending) RC_TRY(ensure_energy(e));
   e->shape = to_shape(shape);
   e->t.nbx = sweep_partials_x(e->g(), e->shape);
@@ -906,8 +907,8 @@ extern "C" int ea_batch_set_shape(ea_batch *b, const ea_shape *shape) {
   RC_TRY(check_shape(shape));
   for (size_t i = 0; i < b->eng.size(); ++i)
     if (!shape_fits(b->eng[i]->g(), to_shape(shape)))
-      return fail(EA_E_ARG, "sweep_kind 2: tube %zu's tile (256 + L*L/16 half sites) exceeds 768",
-                  i);
+      return fail(EA_E_ARG, "sweep_kind %d: tube %zu's tile (256 + L*L/16 half sites) exceeds 768",
+                  shape->sweep_kind, i);
   b->launch = to_shape(shape);
   return EA_OK;
 }

@@ -183,3 +183,23 @@ def test_launch_shapes_with_other_replica_counts(nr):
     for shp in ((32, 512, 0, 1), (64, 1024, 1, 0), (64, 256, 2, 1)):
         say("nr", nr, "shape", shp)
         same(want, sim(shp), "nr %d shape %s" % (nr, shp))
+
+
+def test_tiled_sweeps_at_the_widest_rows():
+    """L=64: the tile (256 + 2*128 half sites from an even origin) needs two
+    16-byte chunks per thread; kind 2 equals the default shape."""
+    L, Lz, nr, nt, ns = 64, 16, 4, 3, 2
+    betas = eatube.betas_linear(0.6, 1.6, nt)
+
+    def sim(shape):
+        x = eatube.Simulation(L, Lz, nr, nt, ns, bc=0, betas=betas,
+                              seeds=(81, 82, 83, 84), job_id=6)
+        if shape:
+            x.engine.set_shape(*shape)
+        x.steps(2, cluster=True)
+        return state(x)
+
+    want = sim(None)
+    for shp in ((64, 1024, 2, 0), (64, 512, 2, 1)):
+        say("L=64 shape", shp)
+        same(want, sim(shp), "L=64 shape %s" % (shp,))
