@@ -1273,26 +1273,44 @@ struct Rows {
 
 // Cluster labelling of a slab of `planes` full z-planes in LDS, all threads
 // of the block taking part.  rows[r] holds the q bits of row r = y + L*z
-// (bit x; L <= 64); lab[i] is the label of local site i = x + L*r.  Label
-// values are local site indices and end as the smallest index of each
-// equal-q cluster:
-//  1. every site points at the first site of its run of equal q along the
-//     periodic x row (a run wrapping past x = L-1 at the row start);
+// (bit x; L <= 64, a multiple of 8); lab[i] is the parent of local site
+// i = x + L*r.  Label values are local site indices; after link_rows, the
+// first site of each run of equal q along x is a node of a forest whose
+// roots are the smallest index of each equal-q cluster, and every other
+// site points at its run's first site:
+//  1. every site points at the first site of its run along the periodic x
+//     row (a run wrapping past x = L-1 belongs to the row start), 8 sites
+//     per thread: the first one's run start is found from the run
+//     boundaries, the others step along them;
 //  2. y and z bonds between rows, from bit masks: a bond at x is united
 //     only where a run overlap starts (when the bond at x-1 joins the same
-//     two runs the union is implied);
-//  3. flatten.
+//     two runs the union is implied).
 // z bonds leave the slab only when wrap_z (the whole periodic tube).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <typename I>
-__device__ void label_rows(const Lattice &g, I *lab, const uint64_t *rows, int planes,
-                           bool wrap_z) {
+__device__ void link_rows(const Lattice &g, I *lab, const uint64_t *rows, int planes,
+                          bool wrap_z) {
+  static_assert(sizeof(I) == 2, "8 sites per 16-byte store");
   using R = Rows<uint64_t>;
   const Forest<I> f{lab};
   const int L = g.L, nrows = planes * L, T = blockDim.x, tid = threadIdx.x;
   const uint64_t full = L == 64 ? ~0ULL : ((1ULL << L) - 1);
-  for (int i = tid; i < nrows * L; i += T) {
-    const int r = qdiv(i, g.mL), x = i - r * L;
-    lab[i] = r * L + R::rep(R::bounds(rows[r], L, full), x);
+  for (int c = tid; 8 * c < nrows * L; c += T) {
+    const int r = qdiv(8 * c, g.mL), x0 = 8 * c - r * L;
+    const uint64_t b = R::bounds(rows[r], L, full);
+    const int hb = (b && !(b & 1)) ? R::hi(b) : L;  // start of a run that wraps
+    uint32_t rep = (uint32_t)R::rep(b, x0), v[4];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int x = x0 + k;
+      if (k > 0 && ((b >> x) & 1ULL)) rep = x == hb ? 0u : (uint32_t)x;
+      const uint32_t l = (uint32_t)(r * L) + rep;
+      if (k & 1)
+        v[k >> 1] |= l << 16;
+      else
+        v[k >> 1] = l;
+    }
+    *reinterpret_cast<u32x4 *>(lab + 8 * c) = u32x4{v[0], v[1], v[2], v[3]};
   }
   __syncthreads();
   {
@@ -1327,8 +1345,6 @@ __device__ void label_rows(const Lattice &g, I *lab, const uint64_t *rows, int p
       f.unite(base + x, base2 + x);
     }
   }
-  __syncthreads();
-  for (int i = tid; i < nrows * L; i += T) lab[i] = f.root(i);
   __syncthreads();
 }
 
@@ -1614,27 +1630,44 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *__restrict__ tube
     grow[r] = q;
   }
   __syncthreads();
-  label_rows(g, lab, rows, planes, false);  // ends with a barrier; rows are not read again
-  // the roots of the sites on a seam plane: plane 0 unless it starts an open
-  // tube, plane planes-1 unless it ends one
-  const bool pbc = g.bc == EA_PBC;
-  const bool below = z0 > 0 || pbc, above = z0 + planes < g.Lz || pbc;
+  link_rows(g, lab, rows, planes, false);  // ends with a barrier; rows are not read again
   uint32_t *seam = reinterpret_cast<uint32_t *>(rows);  // nsite bits fit the rows' words
   for (int w = tid; w < (nsite + 31) / 32; w += T) seam[w] = 0;
   __syncthreads();
-  for (int i = tid; i < S; i += T) {
-    if (below) {
-      const uint32_t r = lab[i];
-      atomicOr(seam + (r >> 5), 1u << (r & 31));
-    }
-    if (above) {
-      const uint32_t r = lab[(planes - 1) * S + i];
-      atomicOr(seam + (r >> 5), 1u << (r & 31));
+  // Every site gets its cluster's root, 8 sites per thread and each run's
+  // root found once, into LDS and HBM.  Root lookups only follow parents,
+  // which are runs' first sites, so overwriting other sites is safe while
+  // other threads still look up roots.  The roots of the sites on a seam
+  // plane are marked: plane 0 unless it starts an open tube, plane
+  // planes-1 unless it ends one.
+  const bool pbc = g.bc == EA_PBC;
+  const bool below = z0 > 0 || pbc, above = z0 + planes < g.Lz || pbc;
+  const auto gl16 = reinterpret_cast<global_t<u32x4> *>(gv(a.hl16) + ib * g.V + gbase);
+  {
+    const Forest<uint16_t> f{lab};
+    for (int c = tid; 8 * c < nsite; c += T) {
+      const u32x4 v = *reinterpret_cast<const u32x4 *>(lab + 8 * c);
+      const bool mark = (below && 8 * c < S) || (above && 8 * c >= nsite - S);
+      uint32_t o[4], prev = ~0u, root = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t l = (v[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+        if (l != prev) {
+          prev = l;
+          root = f.root(l);
+          if (mark) atomicOr(seam + (root >> 5), 1u << (root & 31));
+        }
+        if (k & 1)
+          o[k >> 1] |= root << 16;
+        else
+          o[k >> 1] = root;
+      }
+      const u32x4 w = u32x4{o[0], o[1], o[2], o[3]};
+      *reinterpret_cast<u32x4 *>(lab + 8 * c) = w;
+      gl16[c] = w;
     }
   }
   __syncthreads();
-  const auto gl16 = gv(a.hl16) + ib * g.V + gbase;
-  for (int i = tid; i < nsite; i += T) gl16[i] = lab[i];
   // roots, 32 sites (one coin word) per thread: flip bits of the roots of
   // clusters inside the slab, seam roots as forest nodes
   const auto gpar = gv(a.hpar) + ib * g.V + gbase;
