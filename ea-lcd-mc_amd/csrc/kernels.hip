@@ -1702,13 +1702,15 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *__restrict__ tube
 }
 
 // z bonds between plane z0-1 (or Lz-1 for the periodic wrap) and plane z0,
-// the first plane of the slab, one lane per row y: the two rows' q bits give
-// the overlaps of equal-q runs, and each overlap is united once, at its
+// the first plane of the slab: the two planes' labels are staged in LDS
+// (16-byte loads), then one lane per row y takes the two rows' q bits, which
+// give the overlaps of equal-q runs, and unites each overlap once, at its
 // start, between the two runs' slab-local roots (every site of a run has
 // its run's root); the same pair is not united twice in a row.
 __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
                                                  const uint64_t *__restrict__ map, Phase ph) {
   using R = Rows<uint64_t>;
+  __shared__ __align__(16) uint16_t lu[kHouMaxL * kHouMaxL], ld[kHouMaxL * kHouMaxL];
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
@@ -1716,28 +1718,36 @@ __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
   const int ns = slab_count(a), slab = wh.local % ns;
   const Inst in = instance_of(a, wh.local / ns, wh.step, ph.call);
   const int L = g.L, S = L * L, y = threadIdx.x, z0 = slab * a.planes;
-  if (y >= L) return;  // no barrier follows
   int zl = z0 - 1;
   if (z0 == 0) {
-    if (g.bc != EA_PBC) return;
+    if (g.bc != EA_PBC) return;  // the whole block: no seam below the tube
     zl = g.Lz - 1;
   }
   const size_t ib = instance_index(g, in);
+  const auto l16 = gv(a.hl16) + ib * g.V;
+  {
+    const auto su = reinterpret_cast<const global_t<u32x4> *>(l16 + (size_t)z0 * S);
+    const auto sd = reinterpret_cast<const global_t<u32x4> *>(l16 + (size_t)zl * S);
+    for (int c = y; 8 * c < S; c += blockDim.x) {
+      reinterpret_cast<u32x4 *>(lu)[c] = su[c];
+      reinterpret_cast<u32x4 *>(ld)[c] = sd[c];
+    }
+  }
+  __syncthreads();
+  if (y >= L) return;  // no barrier follows
   const auto grow = gv(a.hrow) + ib * (g.V / L);
-  const auto gl16 = gv(a.hl16) + ib * g.V;
   uint32_t *par = a.hpar + ib * g.V;  // global unions, no LDS
   const uint64_t full = L == 64 ? ~0ULL : ((1ULL << L) - 1);
   const uint64_t qu = grow[z0 * L + y], qd = grow[zl * L + y];
-  const uint64_t lu = ~(qu ^ (qu << 1)) & full & ~1ULL;  // q[x] == q[x-1]
-  const uint64_t ld = ~(qd ^ (qd << 1)) & full & ~1ULL;
-  uint64_t todo = ~(qu ^ qd) & full & ~(lu & ld);
+  const uint64_t lqu = ~(qu ^ (qu << 1)) & full & ~1ULL;  // q[x] == q[x-1]
+  const uint64_t lqd = ~(qd ^ (qd << 1)) & full & ~1ULL;
+  uint64_t todo = ~(qu ^ qd) & full & ~(lqu & lqd);
   const uint32_t bu = (uint32_t)z0 * S, bd = (uint32_t)(zl / a.planes) * a.planes * S;
-  const uint32_t ru = (uint32_t)(z0 * L + y) * L, rd = (uint32_t)(zl * L + y) * L;
   uint32_t pu = ~0u, pd = ~0u;
   while (todo) {
     const int x = R::lo(todo);
     todo &= todo - 1;
-    const uint32_t nu = bu + gl16[ru + x], nd = bd + gl16[rd + x];
+    const uint32_t nu = bu + lu[y * L + x], nd = bd + ld[y * L + x];
     if (nu == pu && nd == pd) continue;
     pu = nu;
     pd = nd;
