@@ -1286,6 +1286,14 @@ struct Rows {
 //     only where a run overlap starts (when the bond at x-1 joins the same
 //     two runs the union is implied).
 // z bonds leave the slab only when wrap_z (the whole periodic tube).
+// Row order of the linking loops: the lanes of a wave take rows nrows/64
+// apart (row rr of the loop is row (rr mod 64) * nrows/64 + rr / 64), so
+// that concurrent unions mostly meet different clusters instead of
+// contending for the same roots.  Identity when 64 does not divide nrows.
+__device__ __forceinline__ int spread_row(int rr, int nrows) {
+  return (nrows & 63) ? rr : (rr & 63) * (nrows >> 6) + (rr >> 6);
+}
+
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <typename I>
 __device__ void link_rows(const Lattice &g, I *lab, const uint64_t *rows, int planes,
@@ -1320,7 +1328,7 @@ __device__ void link_rows(const Lattice &g, I *lab, const uint64_t *rows, int pl
     int base = 0, base2 = 0;
     for (int id = tid;;) {
       while (!todo && id < 2 * nrows) {
-        const int dir = id >= nrows, r = id - dir * nrows;
+        const int dir = id >= nrows, r = spread_row(id - dir * nrows, nrows);
         id += T;
         const int zl = qdiv(r, g.mL), y = r - zl * L;
         int r2;
@@ -1434,7 +1442,8 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   __syncthreads();
 
   const RowT full = (RowT)(L == 64 ? ~0ULL : ((1ULL << L) - 1));
-  for (int r = tid; r < nrows; r += T) {
+  for (int rr = tid; rr < nrows; rr += T) {
+    const int r = spread_row(rr, nrows);
     const RowT q = (RowT)rows[r], b = R::bounds(q, L, full);
     const RowT lq = ~(q ^ (q << 1)) & full & ~RowT(1);  // q[x] == q[x-1]
     const int zl = qdiv(r, g.mL), y = r - zl * L;
