@@ -1570,19 +1570,20 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
 
 // Houdayer for tubes larger than one workgroup's LDS, in slabs of `planes`
 // z-planes; slab block local index: slab + slabs * instance.
-//  1. k_hou_slab labels a slab in LDS (label_rows) and stores every site's
-//     slab-local root (16 bits) and the q bits of every row.  A cluster
-//     that reaches neither of the slab's seam planes ends there: its label
-//     is gbase + its local root, and its flip bit is drawn at once.  The
-//     roots of the clusters that reach a seam plane become nodes of a
-//     forest in HBM (parent = itself).
+//  1. k_hou_slab labels a slab in LDS (link_rows, then one pass of roots)
+//     and stores every site's slab-local root (16 bits) and the q bits of
+//     every row.  A cluster that reaches neither of the slab's seam planes
+//     ends there: its label is gbase + its local root, and its flip bit is
+//     drawn at once.  The roots of the clusters that reach a seam plane
+//     become nodes of a forest in HBM (parent = itself).
 //  2. k_hou_seam unites, across each seam, the nodes of the two slabs' runs
 //     that touch in z.
 //  3. k_hou_apply gives each seam root the coin of its root in that forest,
-//     builds the move mask of every row of the slab run by run, and applies
-//     it to the slab's 16-bit pieces of the two replicas' spin words (piece
-//     bz of a word holds its sites in z block bz, and each slab owns the
-//     pieces of its planes, so blocks never write the same bits).
+//     every site the flip bit of its root, and applies the move to the
+//     slab's 16-bit pieces of the two replicas' spin words (piece bz of a
+//     word holds its sites in z block bz, and each slab owns the pieces of
+//     its planes, so blocks never write the same bits).
+//  4. k_hou_energy recomputes the two changed configurations' energies.
 __device__ __forceinline__ size_t instance_index(const Lattice &g, const Inst &in) {
   return ((size_t)in.grp * g.ns + in.s) * g.ctm + in.beta;
 }
