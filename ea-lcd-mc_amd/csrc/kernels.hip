@@ -430,6 +430,17 @@ __device__ __forceinline__ void update_two(uint64_t *spin, const uint64_t *J, si
   if (s1 != s0) S1[c1 + ls.own] = x1.sv ^ f1;
 }
 
+// Workgroup barrier that waits for this wave's LDS writes only: global loads
+// issued before it stay in flight (__syncthreads waits for all memory
+// traffic).  The signal fences keep the compiler from moving memory
+// accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt, expcnt untouched
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Sum over the first `width` lanes of each width-lane group.
 template <int W>
 __device__ __forceinline__ int group_sum(int e) {
@@ -464,18 +475,10 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *__restrict__ t
   const int tx = threadIdx.x, h0 = b.hb * POS, r = b.r;
   sweep_tables<true>(a, b.beta, wh.step, ph.half, lut, clone_off, ctr);
   __syncthreads();
-  if (tx < 6 * POS) {
-    const int pos = tx / 6, k = tx - 6 * pos;
-    if (h0 + pos < g.n) words[pos][k] = sweep_word(g, ctr, lut, b.beta, h0 + pos, r, k);
-  }
-  __syncthreads();
-  if (tx < POS) thresholds(words[tx], ids[tx][0], ids[tx][1]);
-  __syncthreads();
 
   const int pos = tx & (POS - 1), slot = tx / POS;
   const bool valid = h0 + pos < g.n;
   const int h = valid ? h0 + pos : 0;
-  const uint64_t id1 = ids[pos][0], id2 = ids[pos][1];
   const int p = ph.half & 1;
   const LaneSite ls = lane_site(g, p, h);
   uint64_t *const spin = a.spin;
@@ -486,12 +489,39 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *__restrict__ t
   const uint32_t jpair = 2 * (uint32_t)g.n;
   const uint32_t pstride = (uint32_t)(g.nr * g.nt * nbx);  // partials of one sample
   const uint32_t pbase = (uint32_t)((r * g.nt + b.beta) * nbx + b.hb);
+  // the first pair of samples' loads go out before the random words are
+  // made, so that their latency overlaps that work (the barriers until the
+  // update wait for LDS traffic only)
+  SiteIn x0, x1;
+  auto load_pair = [&](int s0, int s1) {
+    load_lane(gmem(spin + s0 * sstride), gmem(J + s0 * jstride), clone_off[s0 * g.nr + r], jpair,
+              ls, x0);
+    load_lane(gmem(spin + s1 * sstride), gmem(J + s1 * jstride), clone_off[s1 * g.nr + r], jpair,
+              ls, x1);
+  };
+  if (valid && slot < g.ns) load_pair(slot, slot + kSlots < g.ns ? slot + kSlots : slot);
+
+  if (tx < 6 * POS) {
+    const int wp = tx / 6, k = tx - 6 * wp;
+    if (h0 + wp < g.n) words[wp][k] = sweep_word(g, ctr, lut, b.beta, h0 + wp, r, k);
+  }
+  lds_barrier();
+  if (tx < POS) thresholds(words[tx], ids[tx][0], ids[tx][1]);
+  lds_barrier();
+  const uint64_t id1 = ids[pos][0], id2 = ids[pos][1];
+
   for (int s0 = slot; s0 < g.ns; s0 += 2 * kSlots) {
     const int s1 = s0 + kSlots < g.ns ? s0 + kSlots : s0;
     int e0 = 0, e1 = 0;
-    if (valid)
-      update_two(spin, J, sstride, jstride, jpair, s0, s1, clone_off[s0 * g.nr + r],
-                 clone_off[s1 * g.nr + r], ls, p, id1, id2, ph.energy, e0, e1);
+    if (valid) {
+      if (s0 != slot) load_pair(s0, s1);
+      fix_lane(ls, p, x0);
+      fix_lane(ls, p, x1);
+      const uint64_t f0 = site_flip(x0, id1, id2, ph.energy, e0);
+      const uint64_t f1 = site_flip(x1, id1, id2, ph.energy, e1);
+      gmem(spin + s0 * sstride)[clone_off[s0 * g.nr + r] + ls.own] = x0.sv ^ f0;
+      if (s1 != s0) gmem(spin + s1 * sstride)[clone_off[s1 * g.nr + r] + ls.own] = x1.sv ^ f1;
+    }
     if (ph.energy) {
       // every bond has exactly one odd end: the odd half-sweep sees them
       // all; one partial per (sample, replica, beta, block), no atomics
@@ -569,16 +599,6 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_lane(const Tube *__restr
   }
 }
 
-// Workgroup barrier that waits for this wave's LDS writes only: global loads
-// issued before it stay in flight (__syncthreads waits for all memory
-// traffic).  The signal fences keep the compiler from moving memory
-// accesses across it.
-__device__ __forceinline__ void lds_barrier() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt, expcnt untouched
-  __builtin_amdgcn_s_barrier();
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
 
 // One half-sweep, kind 2 (tiled): blocks as kind 1 (256 consecutive half
 // sites of one replica at one beta), but a sample's six neighbour words come
