@@ -645,13 +645,6 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
   const int h0 = b.hb * kLaneThreads, hh = h0 + tx;
   const bool valid = hh < g.n;
   const int h = valid ? hh : 0;
-  uint64_t id1 = 0, id2 = 0;
-  if (valid) {
-    uint32_t w[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] = sweep_word(g, ctr, lut, b.beta, h, r, k);
-    thresholds(w, id1, id2);
-  }
   const int p = ph.half & 1;
   const LaneSite ls = lane_site(g, p, h);
   const uint32_t opp = (uint32_t)((1 - p) * g.half);
@@ -701,9 +694,18 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
     }
   };
 
+  // the first pair's tiles are loaded before the random words are made,
+  // so that their latency overlaps that work
   const int last = g.ns - 1;
   fetch(0, 0);
   fetch(1, min(1, last));
+  uint64_t id1 = 0, id2 = 0;
+  if (valid) {
+    uint32_t w[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = sweep_word(g, ctr, lut, b.beta, h, r, k);
+    thresholds(w, id1, id2);
+  }
   stage(0);
   lds_barrier();
   for (int s0 = 0, buf = 0; s0 < g.ns; s0 += 2, buf ^= 1) {
