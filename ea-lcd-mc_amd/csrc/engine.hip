@@ -119,7 +119,7 @@ struct ea_engine {
     const void *ptrs[] = {t.spin, t.J,    t.wctb, t.lut,  t.ener, t.part,     t.pt,
                           t.acc,  t.att,  t.raw,  t.nd,   t.tab,  t.toff,     t.tlen,
                           t.esum, t.hist, t.hpar, t.hl16, t.hrow, t.hflag,    t.step,
-                          t.ctr,  t.pairs, keep_buf, dt};
+                          t.ctr,  t.pairs, t.ovl,  keep_buf, dt};
     for (const void *p : ptrs)
       if (p) (void)hipFree(const_cast<void *>(p));
     if (stream) (void)hipStreamDestroy(stream);
@@ -646,6 +646,21 @@ extern "C" int ea_energy(ea_engine *e, int32_t *out) {
                            hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
   }
+  return EA_OK;
+}
+
+extern "C" int ea_overlap(ea_engine *e, int32_t *out) {
+  if (!e || !out) return fail(EA_E_ARG, "NULL argument");
+  const Lattice &g = e->g();
+  const size_t cnt = (size_t)g.ns * (g.nr * (g.nr - 1) / 2) * g.nt;
+  if (!e->t.ovl) RC_TRY(dalloc(&e->t.ovl, cnt));
+  RC_TRY(begin_calls(e, 1, 0, 0, 0, 1));  // uploads the descriptor with the buffer
+  {
+    Timed t(e, EA_K_MEASURE);
+    HIP_TRY(launch_overlap(e->dt, e->plan, 0, e->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(out, e->t.ovl, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
   return EA_OK;
 }
 

@@ -819,6 +819,47 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
+// Site overlap Q = sum_i s_i(r1) s_i(r2) of replicas r1 < r2, each at the
+// clone it holds at beta, for every sample, pair and beta: one workgroup per
+// (sample, pair, beta), 64 - 2 popcount(w1 ^ w2) per word of 64 sites,
+// summed by wave reductions.  Output [ns][pair][nt] (pairs in the order
+// (0,1), (0,2), ..., (nr-2, nr-1)).  The reference's program has no such
+// measure; its overlaps are taken from the configuration files offline.
+__global__ void __launch_bounds__(256) k_overlap(const Tube *__restrict__ tubes,
+                                                 const uint64_t *__restrict__ map, int step) {
+  __shared__ int wsum[256 / 64];
+  const Where wh = locate(tubes, map, step);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int np = g.nr * (g.nr - 1) / 2;
+  const int beta = wh.local % g.nt, sp = wh.local / g.nt, s = sp / np;
+  int r1 = 0, k = sp - s * np;
+  while (k >= g.nr - 1 - r1) {
+    k -= g.nr - 1 - r1;
+    ++r1;
+  }
+  const int r2 = r1 + 1 + k;
+  const int c1 = gv(a.wctb)[((size_t)s * g.nr + r1) * g.nt + beta];
+  const int c2 = gv(a.wctb)[((size_t)s * g.nr + r2) * g.nt + beta];
+  const auto base = gmem(a.spin) + (size_t)s * 2 * g.half;
+  const size_t o1 = spin_row(g, r1, c1), o2 = spin_row(g, r2, c2);
+  int d = 0;
+  for (int w = threadIdx.x; w < 2 * g.n; w += blockDim.x) {
+    const int p = w >= g.n, h = w - p * g.n;
+    const size_t off = (size_t)p * g.half + h;
+    d += __popcll(base[o1 + off] ^ base[o2 + off]);
+  }
+  d = wave_sum(d);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += wsum[i];
+    gv(a.ovl)[wh.local] = 128 * g.n - 2 * t;  // V = 128 n sites
+  }
+}
+
 // Sum of the nw partials of each of a chain's nt slots into out[slot]: one
 // lane per slot when the partials are few, the whole wave per slot otherwise.
 template <typename P>
@@ -2011,6 +2052,7 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
     if (shape.sweep_kind == 2) p.tile_words = std::max(p.tile_words, tile_chunks(g));
     add(kMapEnergy, (long long)energy_blocks_host(g) * g.nt * g.ns);
     add(kMapChain, (long long)g.ns * g.nr);
+    add(kMapOverlap, (long long)g.ns * (g.nr * (g.nr - 1) / 2) * g.nt);
     if (!t.cluster || g.ctm <= 0) continue;
     const long long inst = 2LL * g.ns * g.ctm;
     if (t.planes == 0) {
@@ -2127,6 +2169,13 @@ hipError_t launch_energy(const Tube *tubes, const DevPlan &p, int step, hipStrea
   if (p.cnt[kMapChain])
     hipLaunchKernelGGL(k_energy_sum<false>, dim3((unsigned)p.cnt[kMapChain]), dim3(64), 0, s,
                        tubes, p.map(kMapChain), step);
+  return hipGetLastError();
+}
+
+hipError_t launch_overlap(const Tube *tubes, const DevPlan &p, int step, hipStream_t s) {
+  if (p.cnt[kMapOverlap])
+    hipLaunchKernelGGL(k_overlap, dim3((unsigned)p.cnt[kMapOverlap]), dim3(256), 0, s, tubes,
+                       p.map(kMapOverlap), step);
   return hipGetLastError();
 }
 

@@ -73,6 +73,7 @@ struct Tube {
   const int32_t *toff, *tlen;
   double *esum;             // [ns][nr][nt] by beta
   uint8_t *hist;            // [rows][ns][nr][nt] beta of clone
+  int32_t *ovl;             // [ns][replica pair r1 < r2][nt] site overlaps, or NULL
   // slab Houdayer, per instance: parents of the forest of the roots that
   // reach a seam [V], slab-local root of every site [V], q bits of every row
   // [V/L], and per slab the flip bits of the roots whose cluster stays inside
@@ -107,10 +108,11 @@ struct Phase {
 };
 
 // Block maps of every kernel kind over one table of tubes (kMapHouEnergy:
-// one block per cluster instance of the tubes labelled in slabs).
+// one block per cluster instance of the tubes labelled in slabs;
+// kMapOverlap: one per sample, replica pair and beta).
 enum MapKind {
   kMapSweep, kMapEnergy, kMapChain, kMapWhole16, kMapWhole32, kMapWhole64, kMapSlab,
-  kMapSeam, kMapApply, kMapHouEnergy, kMapKinds
+  kMapSeam, kMapApply, kMapHouEnergy, kMapOverlap, kMapKinds
 };
 
 // Launch shape of a table's kernels (chosen by measurement, ea_set_shape /
@@ -174,6 +176,7 @@ void free_plan(DevPlan *d);
 // Launches over a device tube table.  Each returns hipGetLastError().
 hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
 hipError_t launch_energy(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
+hipError_t launch_overlap(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_pt(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
 hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);

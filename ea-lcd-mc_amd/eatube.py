@@ -66,8 +66,8 @@ EXPORTS = [
     "ea_last_error", "ea_device_count", "ea_create", "ea_destroy", "ea_sync",
     "ea_set_keys", "ea_upload_sample", "ea_download_spins",
     "ea_upload_perms", "ea_download_perms", "ea_upload_pt_state",
-    "ea_download_pt_state", "ea_sweep", "ea_energy", "ea_pt", "ea_houdayer",
-    "ea_keep_labels", "ea_download_labels", "ea_run", "ea_read_measures",
+    "ea_download_pt_state", "ea_sweep", "ea_energy", "ea_overlap", "ea_pt",
+    "ea_houdayer", "ea_keep_labels", "ea_download_labels", "ea_run", "ea_read_measures",
     "ea_hq_init", "ea_hq_next", "ea_seeds_from_one", "ea_init_generators",
     "ea_fill_counters", "ea_make_lut", "ea_init_sample",
     "ea_houdayer_pairs", "ea_unpack_lattice", "ea_set_timing",
@@ -111,6 +111,7 @@ def lib():
                                            p(C.c_double), p(C.c_double)]),
         "ea_sweep": (C.c_int, [vp, C.c_int, p(Counter)]),
         "ea_energy": (C.c_int, [vp, p(i32)]),
+        "ea_overlap": (C.c_int, [vp, p(i32)]),
         "ea_pt": (C.c_int, [vp]),
         "ea_houdayer": (C.c_int, [vp, p(i32), u64, p(u64)]),
         "ea_keep_labels": (C.c_int, [vp, C.c_int]),
@@ -327,6 +328,13 @@ class Engine:
     def energy(self):
         out = np.zeros((self.ns, self.nr, self.nt), np.int32)
         check(lib().ea_energy(self.h, _ptr(out, C.c_int32)))
+        return out
+
+    def overlap(self):
+        """site overlaps [ns][replica pair r1 < r2][beta] (ea_overlap)"""
+        npairs = self.nr * (self.nr - 1) // 2
+        out = np.zeros((self.ns, npairs, self.nt), np.int32)
+        check(lib().ea_overlap(self.h, _ptr(out, C.c_int32)))
         return out
 
     def pt(self):

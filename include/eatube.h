@@ -153,6 +153,15 @@ int ea_sweep(ea_engine *e, int nsweeps, const ea_counter *blocks);
  * be NULL: the energies stay on the device for ea_pt). */
 int ea_energy(ea_engine *e, int32_t *out);
 
+/* Site overlaps Q = sum_i s_i(r1) s_i(r2) of every replica pair r1 < r2,
+ * each replica at the clone it holds at beta, into out[ns][pair][nt] with
+ * the pairs in the order (0,1), (0,2), ..., (nr-2, nr-1): popcount
+ * reductions over the MSC words on the device.  The reference's program
+ * (PROGRAM/) has no such measure; its overlaps are computed offline from
+ * the configuration files (io.c:17-280) -- this is the same sum.  The first
+ * call allocates the engine's ns*pairs*nt output words on the device. */
+int ea_overlap(ea_engine *e, int32_t *out);
+
 /* Parallel_Tempering (upd.c:3-48) for every (sample, replica) chain, on the
  * device, from the energies of the last ea_energy / ea_sweep. */
 int ea_pt(ea_engine *e);
@@ -206,8 +215,8 @@ enum {
   EA_K_ENERGY = 1,   /* standalone energy (incl. its zero fill) */
   EA_K_PT = 2,       /* parallel tempering */
   EA_K_HOUDAYER = 3, /* all passes of one cluster move */
-  EA_K_MEASURE = 4,  /* energy sums and history (made inside PT and the
-                        cluster move: no launches of their own) */
+  EA_K_MEASURE = 4,  /* overlaps (the energy sums and history are made
+                        inside PT and the cluster move) */
   EA_K_KINDS = 5
 };
 int ea_set_timing(ea_engine *e, int on);

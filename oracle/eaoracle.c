@@ -1040,6 +1040,36 @@ void orc_sim_energy(orc_sim *s) {
                    s->J + (size_t)12 * n * i, s->ener + (size_t)i * nr * nt);
 }
 
+/* Site overlap Q = sum_i s_i(r1) s_i(r2) of replicas r1 < r2, each at the
+ * clone it holds at beta, for every sample, pair and beta: per word of 64
+ * sites, 64 - 2 popcount(w1 ^ w2).  The reference's program has no such
+ * measure (overlaps are taken from its configuration files offline); this
+ * is that sum over the MSC words of main.cu:1385-1510's layout. */
+void orc_overlap(const orc_sim *s, int32_t *out) {
+  const int nr = s->c.nr, nt = s->c.nt, n = s->n;
+  const size_t half = (size_t)n * nr * nt;
+  const int np = nr * (nr - 1) / 2;
+  for (int i = 0; i < s->c.ns; i++) {
+    int k = 0;
+    for (int r1 = 0; r1 < nr; r1++)
+      for (int r2 = r1 + 1; r2 < nr; r2++, k++)
+        for (int b = 0; b < nt; b++) {
+          const int c1 = s->wctb[((size_t)i * nr + r1) * nt + b];
+          const int c2 = s->wctb[((size_t)i * nr + r2) * nt + b];
+          long q = 0;
+          for (int p = 0; p < 2; p++) {
+            const uint64_t *w = s->spin + 2 * half * i + p * half;
+            for (int h = 0; h < n; h++) {
+              const uint64_t a = w[(size_t)nr * n * c1 + (size_t)nr * h + r1];
+              const uint64_t c = w[(size_t)nr * n * c2 + (size_t)nr * h + r2];
+              q += 64 - 2 * __builtin_popcountll(a ^ c);
+            }
+          }
+          out[((size_t)i * np + k) * nt + b] = (int32_t)q;
+        }
+  }
+}
+
 /* PT loop of main.cu:170-178 */
 void orc_sim_pt(orc_sim *s) {
   const int nr = s->c.nr, nt = s->c.nt;

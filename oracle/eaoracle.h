@@ -36,6 +36,8 @@
  *                                cuRAND): parity unpinned vs the reference
  *   orc_make_lut .............. pinned: byte-identical to the shipped
  *                                LUT_for_PRNG_nbits12_NB40.bin
+ *   orc_overlap ............... pinned: == sum of s1*s2 over the unpacked
+ *                                lattices (orc_unpack_spins, itself pinned)
  */
 #ifndef EAORACLE_H
 #define EAORACLE_H
@@ -206,6 +208,8 @@ void orc_sim_sweeps(orc_sim *s, int nsweeps);
 void orc_sim_energy(orc_sim *s);
 void orc_sim_pt(orc_sim *s);
 void orc_sim_houdayer(orc_sim *s);
+/* Site overlaps [ns][pair r1 < r2][beta] of replicas at each beta. */
+void orc_overlap(const orc_sim *s, int32_t *out);
 /* nsteps of: houfr x (ptfr sweeps, energy, PT), then Houdayer (if on),
  * then energy -- the body of main.cu:149-218 for one `it` with mesfr=1 */
 void orc_sim_steps(orc_sim *s, int nsteps, int ptfr, int houfr, int cluster);

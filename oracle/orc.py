@@ -107,6 +107,7 @@ def lib():
             "orc_sim_energy": (None, [vp]),
             "orc_sim_pt": (None, [vp]),
             "orc_sim_houdayer": (None, [vp]),
+            "orc_overlap": (None, [vp, C.POINTER(C.c_int32)]),
             "orc_sim_steps": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
             "orc_set_threads": (None, [C.c_int]),
         }
@@ -306,6 +307,13 @@ class Sim:
 
     def houdayer(self):
         lib().orc_sim_houdayer(self.h)
+
+    def overlap(self):
+        """site overlaps [ns][pair r1 < r2][beta] (orc_overlap)"""
+        npairs = self.nr * (self.nr - 1) // 2
+        out = np.zeros((self.ns, npairs, self.nt), np.int32)
+        lib().orc_overlap(self.h, ptr(out, C.c_int32))
+        return out
 
     def steps(self, nsteps, ptfr=1, houfr=1, cluster=True):
         lib().orc_sim_steps(self.h, nsteps, ptfr, houfr, int(cluster))

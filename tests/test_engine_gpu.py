@@ -307,3 +307,20 @@ def test_pt_many_temperatures():
         e.steps(chunk, cluster=False)
         o.steps(chunk, cluster=False)
         assert_same_state(o, e, "after %d steps" % chunk)
+
+
+@pytest.mark.parametrize("L,Lz,nr,nt,ns,bc", [(8, 16, 4, 6, 3, 1), (16, 32, 2, 5, 2, 0),
+                                               (16, 264, 4, 3, 1, 0)])
+def test_overlaps_match_oracle(L, Lz, nr, nt, ns, bc):
+    """ea_overlap (popcount reductions on the device) == the oracle's site
+    overlaps of every replica pair at every beta, after steps with PT and
+    cluster moves (when there are two replicas per half)."""
+    o, e = pair(L, Lz, nr, nt, ns, bc=bc, seeds=(31, 32, 33, 34))
+    cl = nr >= 4
+    e.steps(3, cluster=cl)
+    o.steps(3, cluster=cl)
+    got = e.engine.overlap()
+    want = o.overlap()
+    assert got.shape == want.shape == (ns, nr * (nr - 1) // 2, nt)
+    assert np.array_equal(got, want)
+    assert np.all(np.abs(got) <= L * L * Lz)

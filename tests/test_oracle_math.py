@@ -231,3 +231,30 @@ def test_random_word_bits_are_bernoulli():
         per_pos = bits[0].reshape(-1, 64).mean(axis=0)
         npos = bits[0].size // 64
         assert np.all(np.abs(per_pos - p) < 6 * np.sqrt(p * (1 - p) / npos))
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_overlap_is_the_sum_over_unpacked_lattices(bc):
+    """orc_overlap == sum_i s_i(r1) s_i(r2) over the unpacked lattices of
+    replicas r1 < r2 at the clones they hold at each beta."""
+    L, Lz, nr, nt, ns = 8, 16, 4, 5, 3
+    V = L * L * Lz
+    o = orc.Sim(L, Lz, nr, nt, ns, bc=bc, betas=orc.betas_linear(0.6, 1.6, nt),
+                seeds=(5, 6, 7, 8))
+    o.steps(3, cluster=True)
+    got = o.overlap()
+    sp = o.spins()
+    w = o.field("wctb").reshape(ns, nr, nt)
+    assert got.shape == (ns, nr * (nr - 1) // 2, nt)
+    for s in range(ns):
+        uu = orc.unpack_spins(L, Lz, nr, nt, sp[s, 0].ravel(),
+                              sp[s, 1].ravel()).reshape(nr, nt, V).astype(np.int64)
+        k = 0
+        for r1 in range(nr):
+            for r2 in range(r1 + 1, nr):
+                for b in range(nt):
+                    a = 2 * uu[r1, w[s, r1, b]] - 1
+                    c = 2 * uu[r2, w[s, r2, b]] - 1
+                    assert got[s, k, b] == int(np.sum(a * c)), (s, r1, r2, b)
+                k += 1
+    assert np.all(np.abs(got) <= V) and np.any(got != V)
