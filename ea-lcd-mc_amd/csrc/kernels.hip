@@ -861,7 +861,9 @@ __global__ void __launch_bounds__(256) k_overlap(const Tube *__restrict__ tubes,
 }
 
 // Sum of the nw partials of each of a chain's nt slots into out[slot]: one
-// lane per slot when the partials are few, the whole wave per slot otherwise.
+// lane per slot when the partials are few; otherwise a group of g lanes per
+// slot (g a power of two, as many slots at once as the wave holds), each
+// lane adding every g-th partial, then the group's lanes combined.
 template <typename P>
 __device__ __forceinline__ void sum_partials(P part, int nt, int nw, int lane, int32_t *out) {
   if (nw <= 32) {
@@ -871,11 +873,15 @@ __device__ __forceinline__ void sum_partials(P part, int nt, int nw, int lane, i
       out[b] = acc;
     }
   } else {
-    for (int b = 0; b < nt; ++b) {
+    int g = 1;
+    while (g < 64 && 2 * g * nt <= 64) g *= 2;
+    for (int b0 = 0; b0 < nt; b0 += 64 / g) {  // every lane reaches the shuffles
+      const int b = b0 + lane / g, j = lane & (g - 1);
       int acc = 0;
-      for (int k = lane; k < nw; k += 64) acc += part[(size_t)b * nw + k];
-      acc = wave_sum(acc);
-      if (lane == 0) out[b] = acc;
+      if (b < nt)
+        for (int k = j; k < nw; k += g) acc += part[(size_t)b * nw + k];
+      for (int off = 1; off < g; off <<= 1) acc += __shfl_xor(acc, off, 64);
+      if (b < nt && j == 0) out[b] = acc;
     }
   }
 }
