@@ -351,23 +351,20 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   if (!x.val.empty())
     HIP_TRY(hipMemcpy(tab, x.val.data(), sizeof(double) * x.val.size(), hipMemcpyHostToDevice));
 
-  // Houdayer: a tube whose labels fit one workgroup's LDS is labelled there,
-  // larger tubes slab by slab with a merge in HBM
+  // Houdayer: a tube is labelled in one workgroup per instance, or slab by
+  // slab with a merge in HBM (hou_use_slabs)
   if (g.ctm > 0) {
     if (g.L > kHouMaxL)
       return fail(EA_E_ARG, "the Houdayer move supports L <= %d (L=%d)", kHouMaxL, g.L);
-    int lds_block = 0, lds_optin = 0;
+    int lds_block = 0, lds_optin = 0, lds_cu = 0, cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     HIP_TRY(hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, dev));
+    HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                  dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const int lds_max = std::max(lds_block, lds_optin);
-    if (hou_whole_lds(g) > (size_t)lds_max) {
-      // slabs as thick as fit two labelling workgroups into one CU's LDS
-      // (two 1024-thread workgroups fill a CU's wave slots)
-      int lds_cu = 0;
-      HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
-                                    dev));
-      const size_t lds_slab = std::min<size_t>(lds_max, std::max(lds_cu / 2, 1 << 15));
-      t.planes = hou_slab_planes(g, lds_slab);
+    if (hou_use_slabs(g, lds_max, lds_cu, cus)) {
+      t.planes = hou_slab_planes(g, hou_slab_budget(lds_max, lds_cu));
       const size_t inst = (size_t)2 * g.ns * g.ctm;
       const size_t slabs = (g.Lz + t.planes - 1) / t.planes;
       if ((rc = dalloc(&t.hpar, inst * g.V)) || (rc = dalloc(&t.hl16, inst * g.V)) ||
@@ -383,8 +380,8 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
 }
 
 // Device bytes ea_create allocates for cfg, with lds_max bytes of LDS per
-// workgroup deciding the Houdayer path.
-static uint64_t engine_bytes(const ea_config *cfg, int lds_max) {
+// workgroup, lds_cu per CU and cus CUs deciding the Houdayer path.
+static uint64_t engine_bytes(const ea_config *cfg, int lds_max, int lds_cu, int cus) {
   Lattice g{};
   g.L = cfg->L;
   g.Lz = cfg->Lz;
@@ -402,15 +399,15 @@ static uint64_t engine_bytes(const ea_config *cfg, int lds_max) {
                sizeof(int) + sizeof(Tube);
   const ExpTables x = make_exp_tables(cfg->betas, g.nt, g.V);
   b += 8ull * x.nd.size() + 8ull * x.off.size() + 8ull * std::max<size_t>(x.val.size(), 1);
-  if (g.ctm > 0 && hou_whole_lds(g) > (size_t)lds_max)
-    b += hou_slab_bytes(g, hou_slab_planes(g, lds_max / 2));
+  if (g.ctm > 0 && hou_use_slabs(g, lds_max, lds_cu, cus))
+    b += hou_slab_bytes(g, hou_slab_planes(g, hou_slab_budget(lds_max, lds_cu)));
   return b;
 }
 
 extern "C" int ea_footprint(const ea_config *cfg, uint64_t *bytes) {
   if (!bytes) return fail(EA_E_ARG, "bytes is NULL");
   RC_TRY(check_config(cfg));
-  *bytes = engine_bytes(cfg, 160 * 1024);
+  *bytes = engine_bytes(cfg, 160 * 1024, 160 * 1024, 256);  // MI355X: 160 KB LDS, 256 CUs
   return EA_OK;
 }
 

@@ -2030,6 +2030,16 @@ size_t hou_apply_lds(const Lattice &g, int planes) {
          2 * 2 * (size_t)g.L * g.L;
 }
 
+bool hou_use_slabs(const Lattice &g, int lds_max, int lds_cu, int cus) {
+  const size_t whole = hou_whole_lds(g);
+  if (whole > (size_t)lds_max) return true;
+  return whole > (size_t)lds_cu / 2 && 2LL * g.ns * g.ctm < cus;
+}
+
+size_t hou_slab_budget(int lds_max, int lds_cu) {
+  return std::min<size_t>(lds_max, std::max(lds_cu / 2, 1 << 15));
+}
+
 int hou_slab_planes(const Lattice &g, size_t lds_slab) {
   int planes = std::max(1, std::min(g.Lz, kHouSlabSites / (g.L * g.L)));
   while (planes > 1 && hou_slab_lds(g, planes) > lds_slab) --planes;

@@ -186,12 +186,21 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
 
 // LDS bytes of the whole-tube and slab Houdayer kernels; a tube is labelled
-// in one workgroup when hou_whole_lds() fits the device's per-block LDS,
-// otherwise in slabs of about kHouSlabSites sites.  Both need L <= 64.
+// in one workgroup or in slabs as hou_use_slabs decides.  Both need
+// L <= 64.
 size_t hou_whole_lds(const Lattice &g);
 size_t hou_slab_lds(const Lattice &g, int planes);
 size_t hou_apply_lds(const Lattice &g, int planes);
 constexpr int kHouSlabSites = 65536;  // slab-local labels are 16-bit
+// Whether a tube's cluster moves take the slab path: always when its
+// whole-tube labelling does not fit a workgroup's LDS (lds_max); also when
+// it takes more than half a CU's LDS (lds_cu) and the tube has fewer
+// instances than the device has CUs (cus), where the longest instance, not
+// their number, sets the kernel's time and half-size slabs finish sooner.
+bool hou_use_slabs(const Lattice &g, int lds_max, int lds_cu, int cus);
+// LDS bytes of one slab's labelling: as much as lets two labelling
+// workgroups share a CU (two 1024-thread workgroups fill its wave slots).
+size_t hou_slab_budget(int lds_max, int lds_cu);
 // Planes per slab: the thickest slab whose labelling fits lds_slab bytes.
 int hou_slab_planes(const Lattice &g, size_t lds_slab);
 // 64-bit words of one slab's flip (or seam-root) bits.

@@ -324,3 +324,18 @@ def test_overlaps_match_oracle(L, Lz, nr, nt, ns, bc):
     assert got.shape == want.shape == (ns, nr * (nr - 1) // 2, nt)
     assert np.array_equal(got, want)
     assert np.all(np.abs(got) <= L * L * Lz)
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_slab_path_for_tubes_with_few_instances(bc):
+    """L=24 Lz=96 fits one workgroup's LDS but not half a CU's; with one
+    sample (4 instances, fewer than the CUs) it is labelled in slabs, and
+    still equals the oracle (labels, spins, energies, PT)."""
+    o, e = pair(24, 96, 4, 4, 1, bc=bc, seeds=(41, 42, 43, 44))
+    o.keep_labels()
+    e.engine.keep_labels()
+    e.steps(2, cluster=True)
+    o.steps(2, cluster=True)
+    assert_same_state(o, e, "24x96")
+    assert np.array_equal(e.engine.download_labels().ravel(),
+                          o.field("last_labels"))
