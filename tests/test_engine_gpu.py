@@ -339,3 +339,33 @@ def test_slab_path_for_tubes_with_few_instances(bc):
     assert_same_state(o, e, "24x96")
     assert np.array_equal(e.engine.download_labels().ravel(),
                           o.field("last_labels"))
+
+
+def _device_free():
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+@pytest.mark.parametrize("L,Lz,ns", [(24, 96, 8), (24, 96, 31), (16, 264, 4)])
+def test_footprint_matches_allocation(L, Lz, ns):
+    """ea_footprint (the serving loop's admission figure) agrees with what
+    ea_create allocates, on both Houdayer paths: 24x96 with 8 samples (192
+    instances, fewer than the CUs) is labelled in slabs, whose buffers
+    (~64 MB) dominate; with 31 samples in one workgroup per instance."""
+    nr, nt = 4, 24
+    betas = eatube.betas_linear(0.6, 1.6, nt)
+    want = eatube.footprint(L, Lz, nr, nt, ns, 1, betas)
+    warm = eatube.Simulation(8, 16, nr, nt, 1, bc=1, betas=betas, seeds=(1, 2, 3, 4))
+    warm.engine.sync()  # the runtime's context and code objects are loaded
+    warm.engine.close()
+    before = _device_free()
+    sim = eatube.Simulation(L, Lz, nr, nt, ns, bc=1, betas=betas, seeds=(1, 2, 3, 4))
+    sim.engine.sync()
+    used = before - _device_free()
+    say("footprint", want, "allocated", used)
+    # allocations are rounded to pages, and each engine adds a few small
+    # runtime buffers
+    assert want * 0.95 - (16 << 20) <= used <= want * 1.05 + (16 << 20)
+    sim.engine.close()
