@@ -22,6 +22,10 @@ constexpr double kFnorm = 5.4210108624275218e-20;  // random_generator.h:92
 
 // ---------------------------------------------------------------- helpers
 
+// 16-byte vectors for wide loads and stores
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
 // Global-memory view of the tube's arrays: the table holds generic
 // pointers, and loads through them would compile to flat loads, whose waits
 // also drain LDS traffic and serialise the samples' loads.
@@ -256,7 +260,6 @@ __device__ __forceinline__ void sweep_tables(const Tube &a, int beta, int step, 
   const Lattice &g = a.g;
   const int tx = threadIdx.x, T = blockDim.x;
   const auto src = gv(reinterpret_cast<const uint4 *>(a.lut + (size_t)beta * EA_LUT_BYTES));
-  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   u32x4 *dst = reinterpret_cast<u32x4 *>(lut);
   const auto vsrc = reinterpret_cast<const global_t<u32x4> *>(src);
   for (int i = tx; i < EA_LUT_BYTES / 16; i += T) dst[i] = vsrc[i];
@@ -678,7 +681,6 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
   const uint32_t jpair = 2 * (uint32_t)g.n;
 
   // row offsets are even (n is), so a chunk is 16-byte aligned
-  typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
   u64x2 tv[2][KC];
   auto fetch = [&](int slot, int s) {
     const gword *S = gmem(spin + s * sstride) + clone_off[s * g.nr + r] + opp;
@@ -1363,7 +1365,6 @@ __device__ __forceinline__ int spread_row(int rr, int nrows) {
   return (nrows & 63) ? rr : (rr & 63) * (nrows >> 6) + (rr >> 6);
 }
 
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <typename I>
 __device__ void link_rows(const Lattice &g, I *lab, const uint64_t *rows, int planes,
                           bool wrap_z) {
@@ -1758,7 +1759,6 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *__restrict__ tube
     if (32 * w < nsite) {
       const uint32_t coins = coin_run(a.coin_seed, cidx + 32 * w), sb = seam[w];
       // the group's labels in four 16-byte reads (lanes 64 bytes apart)
-      typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
       const u32x4 *lv = reinterpret_cast<const u32x4 *>(lab + 32 * w);
       u32x4 v[4];
 #pragma unroll
@@ -1838,7 +1838,6 @@ __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
 // seam roots[2*L*L] u16.
 __global__ void __launch_bounds__(256) k_hou_apply(const Tube *__restrict__ tubes,
                                                    const uint64_t *__restrict__ map, Phase ph) {
-  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   extern __shared__ uint64_t lds[];
   __shared__ int nseam;
   const Where wh = locate(tubes, map, ph.step);
