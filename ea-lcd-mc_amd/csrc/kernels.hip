@@ -1785,11 +1785,15 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *__restrict__ tube
 // (16-byte loads), then one lane per row y takes the two rows' q bits, which
 // give the overlaps of equal-q runs, and unites each overlap once, at its
 // start, between the two runs' slab-local roots (every site of a run has
-// its run's root); the same pair is not united twice in a row.
+// its run's root).  A pair of roots is united once per seam: repeats in a
+// row are skipped, and a set in LDS lets only the first lane to meet a pair
+// unite it (the same clusters touch across a seam in many rows).
 __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
                                                  const uint64_t *__restrict__ map, Phase ph) {
   using R = Rows<uint64_t>;
   __shared__ __align__(16) uint16_t lu[kHouMaxL * kHouMaxL], ld[kHouMaxL * kHouMaxL];
+  constexpr int kSet = 512;  // pairs already united by this seam (open addressing)
+  __shared__ uint64_t seen[kSet];
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
@@ -1811,6 +1815,7 @@ __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
       reinterpret_cast<u32x4 *>(lu)[c] = su[c];
       reinterpret_cast<u32x4 *>(ld)[c] = sd[c];
     }
+    for (int i = y; i < kSet; i += blockDim.x) seen[i] = ~0ULL;
   }
   __syncthreads();
   if (y >= L) return;  // no barrier follows
@@ -1830,7 +1835,20 @@ __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
     if (nu == pu && nd == pd) continue;
     pu = nu;
     pd = nd;
-    unite(par, nu, nd);
+    // the first lane to meet a pair unites it; a full set just unites
+    const uint64_t key = (uint64_t)nu << 32 | nd;
+    uint32_t hsh = (uint32_t)((key * 0x9E3779B97F4A7C15ULL) >> 55);
+    bool fresh = true;
+    for (int t = 0; t < kSet; ++t, hsh = (hsh + 1) & (kSet - 1)) {
+      const unsigned long long old =
+          atomicCAS(reinterpret_cast<unsigned long long *>(seen + hsh), ~0ULL, key);
+      if (old == ~0ULL) break;
+      if (old == key) {
+        fresh = false;
+        break;
+      }
+    }
+    if (fresh) unite(par, nu, nd);
   }
 }
 
