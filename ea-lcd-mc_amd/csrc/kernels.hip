@@ -1250,23 +1250,35 @@ __device__ __forceinline__ void unite(uint32_t *lab, uint32_t a, uint32_t b) {
 // LDS, and the accesses stay LDS instructions.  Every parent index is below
 // its child's, so the forest has no cycles; a node that is not a root only
 // ever gets re-pointed at an ancestor.
-template <typename I = uint32_t>
+template <typename I = uint32_t, bool Swz = false>
 struct Forest {
   I *p;
+  // LDS slot of entry i.  With Swz, bits 8-13 of i are folded into the bits
+  // that pick the LDS bank (bits 1-6 for 16-bit entries, 0-5 for 32-bit
+  // ones), so entries a multiple of 256 apart sit in different banks.  The
+  // map is its own inverse, keeps every 128-entry block in place (the forest
+  // size is a multiple of 128) and keeps the two 16-bit entries of a 32-bit
+  // word together.
+  static __device__ __forceinline__ uint32_t ix(uint32_t i) {
+    if constexpr (Swz)
+      return i ^ (((i >> 8) & 63u) << (sizeof(I) == 2 ? 1 : 0));
+    else
+      return i;
+  }
   __device__ __forceinline__ uint32_t get(uint32_t i) const {
-    return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __hip_atomic_load(p + ix(i), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   __device__ __forceinline__ void put(uint32_t i, uint32_t v) const {
-    __hip_atomic_store(p + i, (I)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(p + ix(i), (I)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   // p[a] = min(p[a], b), returning the previous p[a].  16-bit parents sit
   // two to a 32-bit LDS word and are lowered by compare-and-swap of the
   // word (LDS has no 16-bit atomic minimum).
   __device__ __forceinline__ uint32_t lower(uint32_t a, uint32_t b) const {
     if constexpr (sizeof(I) == 4) {
-      return atomicMin(p + a, b);
+      return atomicMin(p + ix(a), b);
     } else {
-      uint32_t *w = reinterpret_cast<uint32_t *>(p) + (a >> 1);
+      uint32_t *w = reinterpret_cast<uint32_t *>(p) + (ix(a) >> 1);
       const int sh = (a & 1) * 16;
       uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       for (;;) {
@@ -1495,7 +1507,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   uint64_t *qw = rows + nrows;  // q words, then replica r1's new words
   uint64_t *nw = qw + 2 * g.n;  // replica r2's new words
   uint32_t *cb = reinterpret_cast<uint32_t *>(nw + 2 * g.n);
-  const Forest<IdxT> f{reinterpret_cast<IdxT *>(cb + V / 32)};
+  const Forest<IdxT, true> f{reinterpret_cast<IdxT *>(cb + V / 32)};
   __shared__ int esum[2];
   const int beta = in.beta, s = in.s, grp = in.grp;
   const Pair pw = pair_words(g, a, in.pairs, s, grp, beta);
@@ -1505,7 +1517,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
     const size_t o = (size_t)p * g.half + h;
     qw[w] = spin[pw.o1 + o] ^ spin[pw.o2 + o];
   }
-  for (int i = tid; i < V; i += T) f.p[i] = i;
+  for (int i = tid; i < V; i += T) f.p[i] = f.ix(i);  // every entry its own root
   __syncthreads();
   for (int r = tid; r < nrows; r += T)
     rows[r] = row_bits(g, r, [&](int p, int h) { return qw[p * g.n + h]; });
