@@ -862,30 +862,34 @@ __global__ void __launch_bounds__(256) k_overlap(const Tube *__restrict__ tubes,
   }
 }
 
-// Sum of the nw partials of each of a chain's nt slots into out[slot]: one
-// lane per slot when the partials are few; otherwise a group of g lanes per
-// slot (g a power of two, as many slots at once as the wave holds), each
-// lane adding every g-th partial, then the group's lanes combined.
+// Adds the nw partials of each of a chain's nt slots (slot-major) to
+// out[slot], which the caller has zeroed: lane l takes the l-th of 64
+// contiguous ranges of the nt*nw partials, loads them 8 at a time with all 8
+// loads in flight, and adds each run of one slot to out[slot] with one LDS
+// atomic.  Integer sums, so the order of the atomics does not matter.  The
+// caller synchronises before reading out[].
 template <typename P>
 __device__ __forceinline__ void sum_partials(P part, int nt, int nw, int lane, int32_t *out) {
-  if (nw <= 32) {
-    for (int b = lane; b < nt; b += 64) {
-      int acc = 0;
-      for (int k = 0; k < nw; ++k) acc += part[(size_t)b * nw + k];
-      out[b] = acc;
-    }
-  } else {
-    int g = 1;
-    while (g < 64 && 2 * g * nt <= 64) g *= 2;
-    for (int b0 = 0; b0 < nt; b0 += 64 / g) {  // every lane reaches the shuffles
-      const int b = b0 + lane / g, j = lane & (g - 1);
-      int acc = 0;
-      if (b < nt)
-        for (int k = j; k < nw; k += g) acc += part[(size_t)b * nw + k];
-      for (int off = 1; off < g; off <<= 1) acc += __shfl_xor(acc, off, 64);
-      if (b < nt && j == 0) out[b] = acc;
+  const int total = nt * nw, per = (total + 63) / 64;
+  const int e0 = min(lane * per, total), e1 = min(e0 + per, total);
+  if (e0 >= e1) return;
+  int b = e0 / nw, next = (b + 1) * nw, sum = 0;
+  for (int e = e0; e < e1; e += 8) {
+    int v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = e + u < e1 ? (int)part[e + u] : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (e + u < e1 && e + u == next) {  // a new slot starts here
+        atomicAdd(out + b, sum);
+        sum = 0;
+        ++b;
+        next += nw;
+      }
+      sum += v[u];
     }
   }
+  atomicAdd(out + b, sum);
 }
 
 // ener[chain][c] from the chain's partials: the sweep's partials by beta
@@ -899,6 +903,8 @@ __global__ void __launch_bounds__(64) k_energy_sum(const Tube *__restrict__ tube
   if (wh.step >= a.nsteps) return;
   const int chain = wh.local, nt = a.g.nt, lane = threadIdx.x;
   const int nw = BY_BETA ? a.nbx : 4 * energy_blocks_x(a.g);
+  for (int b = lane; b < nt; b += 64) E[b] = 0;
+  __syncthreads();
   sum_partials(a.part + (size_t)chain * nt * nw, nt, nw, lane, E);
   __syncthreads();
   for (int b = lane; b < nt; b += 64) {
@@ -955,36 +961,68 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
   const auto ener = gv(a.ener) + (size_t)chain * nt;
   const auto esum = gv(a.esum) + (size_t)chain * nt;
   // everything the serial pass reads is staged before it: no global load
-  // sits on the chain of dependent swap decisions
-  const uint64_t zseed = st->zseed;
-  const uint8_t ip = st->ip, ip1 = st->ip1, ip2 = st->ip2, ip3 = st->ip3;
-  for (int i = lane; i < 256; i += 64) ira[i] = st->ira[i];
-  for (int i = lane; i < nt; i += 64) perm[i] = wctb[i];
+  // sits on the chain of dependent swap decisions.  The loads of every array
+  // are issued into registers before the first of them is stored to LDS, so
+  // the staging costs one memory round trip rather than one per array.
+  constexpr int kPer = (EA_MAX_BETAS + 63) / 64;  // slots per lane
+  const bool measure = ph.measure, energy = ph.energy;
   const size_t k0 = (size_t)chain * np;
   const auto acc = gv(a.acc) + k0, att = gv(a.att) + k0, raw = gv(a.raw) + k0;
-  for (int i = lane; i < np; i += 64) {
-    ndv[i] = gv(a.nd)[i];
-    stlen[i] = gv(a.tlen)[i];
-    stoff[i] = gv(a.toff)[i];
-    sacc[i] = acc[i];
-    satt[i] = att[i];
-    sraw[i] = raw[i];
+  const uint64_t zseed = st->zseed;
+  const uint8_t ip = st->ip, ip1 = st->ip1, ip2 = st->ip2, ip3 = st->ip3;
+  uint64_t r_ira[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) r_ira[j] = st->ira[lane + 64 * j];
+  uint32_t r_perm[kPer];
+  int32_t r_en[kPer], r_tlen[kPer], r_toff[kPer];
+  double r_es[kPer], r_nd[kPer], r_acc[kPer], r_att[kPer], r_raw[kPer];
+  // (no branches around the loads: indexes past an array's end read its
+  // last element, nt >= 2 so every array has one, and are not stored)
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int i = lane + 64 * j, it = min(i, nt - 1), ip_ = min(i, np - 1);
+    r_perm[j] = wctb[it];
+    r_es[j] = esum[it];
+    r_en[j] = ener[it];
+    r_nd[j] = gv(a.nd)[ip_];
+    r_tlen[j] = gv(a.tlen)[ip_];
+    r_toff[j] = gv(a.toff)[ip_];
+    r_acc[j] = acc[ip_];
+    r_att[j] = att[ip_];
+    r_raw[j] = raw[ip_];
   }
-  const bool measure = ph.measure;
-  if (measure)
-    for (int b = lane; b < nt; b += 64) sesum[b] = esum[b];
-  if (ph.energy) {
-    // the sweep left partial energies by beta: dE holds them by beta for a
+  if (energy) {
+    // the sweep left partial energies by beta: dE sums them by beta for a
     // moment, E receives them by clone
+    for (int b = lane; b < nt; b += 64) dE[b] = 0;
+    lds_barrier();  // the staging loads stay in flight
     sum_partials(gv(a.part) + (size_t)chain * nt * a.nbx, nt, a.nbx, lane, dE);
-    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) ira[lane + 64 * j] = r_ira[j];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int i = lane + 64 * j;
+    if (i < nt) {
+      perm[i] = (uint8_t)r_perm[j];
+      if (measure) sesum[i] = r_es[j];
+      if (!energy) E[i] = r_en[j];
+    }
+    if (i < np) {
+      ndv[i] = r_nd[j];
+      stlen[i] = r_tlen[j];
+      stoff[i] = r_toff[j];
+      sacc[i] = r_acc[j];
+      satt[i] = r_att[j];
+      sraw[i] = r_raw[j];
+    }
+  }
+  __syncthreads();
+  if (energy) {
     for (int b = lane; b < nt; b += 64) {
       E[perm[b]] = dE[b];
       ener[perm[b]] = dE[b];
     }
-    __syncthreads();
-  } else {
-    for (int c = lane; c < nt; c += 64) E[c] = ener[c];
     __syncthreads();
   }
   for (int p = lane; p < nt; p += 64) Ep[p] = E[perm[p]];
