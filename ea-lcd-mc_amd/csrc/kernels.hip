@@ -1092,27 +1092,61 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
     dmin[k] = dm;
   }
   __syncthreads();
-  if (lane == 0) {
-    // the clone moving up the ladder and its energy stay in registers:
-    // position i+1 still holds its clone from before the walk, so no LDS
-    // access depends on an earlier decision
+  if (!exact_walk) {
+    // every decision is d >= dmin[i]: the walk runs in scalar registers, 64
+    // pairs at a time.  Lane k holds pair base+k's clone and energy above it
+    // and its threshold, which the walk reads with readlane, and keeps pair
+    // base+k's outcome, stored once the chunk's steps are made.  The clone
+    // moving up the ladder and its energy stay in registers.
+    int pc = __builtin_amdgcn_readfirstlane(perm[0]);
+    int32_t ec = __builtin_amdgcn_readfirstlane(Ep[0]);
+    for (int base = 0; base < np; base += 64) {
+      const int i = base + lane, cnt = min(64, np - base);
+      const int vpb = i < np ? perm[i + 1] : 0;
+      const int32_t veb = i < np ? Ep[i + 1] : 0, vdm = i < np ? dmin[i] : 0;
+      int32_t myd = 0;
+      int mytake = 0, mykeep = 0;
+      for (int k = 0; k < cnt; ++k) {
+        const int pb = __builtin_amdgcn_readlane(vpb, k);
+        const int32_t eb = __builtin_amdgcn_readlane(veb, k);
+        const int d = eb - ec;
+        const bool take = d >= __builtin_amdgcn_readlane(vdm, k);
+        if (lane == k) {
+          myd = d;
+          mytake = take;
+          mykeep = take ? pb : pc;  // pc moves to position i+1 when taken
+        }
+        if (!take) {
+          pc = pb;
+          ec = eb;
+        }
+      }
+      __syncthreads();  // the chunk's reads of perm are done
+      if (i < np) {
+        dE[i] = myd;
+        took[i] = (uint8_t)mytake;
+        perm[i] = (uint8_t)mykeep;
+      }
+      __syncthreads();
+    }
+    if (lane == 0) perm[np] = (uint8_t)pc;
+  } else if (lane == 0) {
+    // the exact rule, pair by pair: position i+1 still holds its clone from
+    // before the walk, so no LDS access depends on an earlier decision
     int pc = perm[0];
     int32_t ec = Ep[0];
     for (int i = 0; i < np; ++i) {
       const int pb = perm[i + 1];
       const int32_t eb = Ep[i + 1];
       const int d = eb - ec;
-      bool take = d >= dmin[i];
-      if (exact_walk) {
-        const double pt = ndv[i] * (double)d;
-        take = true;
-        if (!(pt >= 0)) {
-          const double u = uu[i], l = lu[i];
-          if (u == 0.0 || fabs(l - pt) <= 1e-12)
-            take = u < exact_exp(a, i, d);
-          else
-            take = l < pt;
-        }
+      const double pt = ndv[i] * (double)d;
+      bool take = true;
+      if (!(pt >= 0)) {
+        const double u = uu[i], l = lu[i];
+        if (u == 0.0 || fabs(l - pt) <= 1e-12)
+          take = u < exact_exp(a, i, d);
+        else
+          take = l < pt;
       }
       dE[i] = d;
       took[i] = take;
@@ -1125,6 +1159,8 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
       }
     }
     perm[np] = (uint8_t)pc;
+  }
+  if (lane == 0) {
     st->ip = (uint8_t)(ip + np);
     st->ip1 = (uint8_t)(ip1 + np);
     st->ip2 = (uint8_t)(ip2 + np);
