@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -255,6 +256,14 @@ static int ensure_hist(ea_engine *e, int rows) {
   return EA_OK;
 }
 
+// EA_PT_EXACT_WALK=1: every PT walk applies the exact rule pair by pair (the
+// path a walk takes when a pair's swap threshold does not verify).  Same
+// decisions either way; it lets that path be tested.
+static int pt_exact_from_env() {
+  const char *v = std::getenv("EA_PT_EXACT_WALK");
+  return v && v[0] == '1' && v[1] == '\0';
+}
+
 extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   if (!out) return fail(EA_E_ARG, "out is NULL");
   *out = nullptr;
@@ -295,6 +304,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   t.nbx = sweep_partials_x(g, e->shape);
   t.frec = 1;
   t.cluster = g.ctm > 0 && g.nr >= 4;
+  t.pt_exact = pt_exact_from_env();
   e->job = cfg->job_id;
   e->betas.assign(cfg->betas, cfg->betas + cfg->nt);
 

@@ -1049,7 +1049,7 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
     }
     __syncthreads();
   }
-  if (lane == 0) exact_walk = 0;
+  if (lane == 0) exact_walk = a.pt_exact;
   for (int k = lane; k < np; k += 64) {
     const uint64_t fin = zs[k] + prs[k];
     const double u = kFnorm * (double)fin;

@@ -85,6 +85,7 @@ struct Tube {
   int planes;               // slab Houdayer planes; 0: whole tube in one block
   int nbx;                  // sweep blocks per beta
   int cluster;              // Houdayer moves are part of this tube's steps
+  int pt_exact;             // every PT walk applies the exact rule pair by pair
   // calls in flight
   int *step;                // device step counter
   int nsteps;               // blocks of a tube with *step >= nsteps do nothing

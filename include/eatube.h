@@ -103,7 +103,11 @@ int ea_select_device(int device);
 
 /* Allocate device state and upload constants: replaces CopyConstOnDevice /
  * CopyDataOnDevice (main.cu:634-725) and ClusterInit (HoudayerXEA.cu:123-181).
- * Binds the calling thread's current HIP device. */
+ * Binds the calling thread's current HIP device.  With EA_PT_EXACT_WALK=1 in
+ * the environment at creation, every PT call of the engine (and of batches
+ * made from it) decides each swap by the exact rule pair by pair, the path
+ * otherwise taken only when a pair's swap threshold does not verify; the
+ * decisions are the same either way, so the switch exists to test that path. */
 int ea_create(const ea_config *cfg, ea_engine **out);
 
 /* Device bytes ea_create(cfg) allocates on gfx950 (160 KiB of LDS per

@@ -309,6 +309,23 @@ def test_pt_many_temperatures():
         assert_same_state(o, e, "after %d steps" % chunk)
 
 
+@pytest.mark.parametrize("case", [
+    dict(L=8, Lz=16, nr=4, nt=16, ns=5, bc=1, tmin=0.7, tmax=1.6, seeds=(11, 12, 13, 14)),
+    dict(L=8, Lz=8, nr=2, nt=220, ns=2, bc=0, tmin=0.4, tmax=2.0, seeds=(61, 62, 63, 64)),
+])
+def test_exact_pt_walk_matches_oracle(case, monkeypatch):
+    """Every PT walk forced onto the exact rule pair by pair (the path a walk
+    takes when a pair's swap threshold does not verify): the same swaps,
+    generator states and statistics as the oracle's."""
+    monkeypatch.setenv("EA_PT_EXACT_WALK", "1")
+    o, e = pair(**case)
+    cluster = case["nr"] >= 4
+    for chunk in (1, 3):
+        e.steps(chunk, cluster=cluster)
+        o.steps(chunk, cluster=cluster)
+        assert_same_state(o, e, "exact walk, after %d steps" % chunk)
+
+
 @pytest.mark.parametrize("L,Lz,nr,nt,ns,bc", [(8, 16, 4, 6, 3, 1), (16, 32, 2, 5, 2, 0),
                                                (16, 264, 4, 3, 1, 0)])
 def test_overlaps_match_oracle(L, Lz, nr, nt, ns, bc):
