@@ -1877,13 +1877,17 @@ __global__ void __launch_bounds__(1024) k_hou_slab(const Tube *__restrict__ tube
 __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
                                                  const uint64_t *__restrict__ map, Phase ph) {
   using R = Rows<uint64_t>;
-  __shared__ __align__(16) uint16_t lu[kHouMaxL * kHouMaxL], ld[kHouMaxL * kHouMaxL];
+  // dynamic LDS: the two planes' 16-bit labels, L*L each (sized for the
+  // widest tube of the launch, not for the largest L, so that more seams
+  // share a CU)
+  extern __shared__ __align__(16) uint16_t seam_lab[];
   constexpr int kSet = 512;  // pairs already united by this seam (open addressing)
   __shared__ uint64_t seen[kSet];
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
   const Lattice g = a.g;
+  uint16_t *const lu = seam_lab, *const ld = seam_lab + g.L * g.L;
   const int ns = slab_count(a), slab = wh.local % ns;
   const Inst in = instance_of(a, wh.local / ns, wh.step, ph.call);
   const int L = g.L, S = L * L, y = threadIdx.x, z0 = slab * a.planes;
@@ -2193,6 +2197,7 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
       add(kMapHouEnergy, inst);
       p.lds_slab = std::max(p.lds_slab, hou_slab_lds(g, t.planes));
       p.lds_apply = std::max(p.lds_apply, hou_apply_lds(g, t.planes));
+      p.lds_seam = std::max(p.lds_seam, (size_t)4 * g.L * g.L);  // two planes of 16-bit labels
     }
   }
   return p;
@@ -2211,6 +2216,7 @@ hipError_t upload_plan(const Plan &p, DevPlan *d) {
   d->lds64 = p.lds64;
   d->lds_slab = p.lds_slab;
   d->lds_apply = p.lds_apply;
+  d->lds_seam = p.lds_seam;
   d->tile_words = p.tile_words;
   d->shape = p.shape;
   d->identity = p.ntubes == 1;
@@ -2348,7 +2354,7 @@ hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStr
     allow_lds(reinterpret_cast<const void *>(k_hou_slab), p.lds_slab);
     hipLaunchKernelGGL(k_hou_slab, dim3((unsigned)p.cnt[kMapSlab]), dim3(1024), p.lds_slab, s,
                        tubes, p.map(kMapSlab), ph);
-    hipLaunchKernelGGL(k_hou_seam, dim3((unsigned)p.cnt[kMapSeam]), dim3(64), 0, s, tubes,
+    hipLaunchKernelGGL(k_hou_seam, dim3((unsigned)p.cnt[kMapSeam]), dim3(64), p.lds_seam, s, tubes,
                        p.map(kMapSeam), ph);
     allow_lds(reinterpret_cast<const void *>(k_hou_apply), p.lds_apply);
     hipLaunchKernelGGL(k_hou_apply, dim3((unsigned)p.cnt[kMapApply]), dim3(256), p.lds_apply, s,

@@ -139,7 +139,7 @@ constexpr int kSweepPosMin = 32;
 struct Plan {
   std::vector<uint64_t> map[kMapKinds];
   Shape shape;
-  size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0;
+  size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0, lds_seam = 0;
   int tile_words = 0;  // 16-byte tile chunks per thread of a tiled sweep (the widest tube's)
   int ntubes = 0;
 };
@@ -151,7 +151,7 @@ struct DevPlan {
   uint64_t *buf = nullptr;
   size_t off[kMapKinds] = {};
   size_t cnt[kMapKinds] = {};
-  size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0;
+  size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0, lds_seam = 0;
   int tile_words = 0;  // 16-byte tile chunks per thread of a tiled sweep (the widest tube's)
   Shape shape;
   bool identity = false;
