@@ -1881,7 +1881,7 @@ __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
   // widest tube of the launch, not for the largest L, so that more seams
   // share a CU)
   extern __shared__ __align__(16) uint16_t seam_lab[];
-  constexpr int kSet = 512;  // pairs already united by this seam (open addressing)
+  constexpr int kSetBits = 8, kSet = 1 << kSetBits;  // pairs already united by this seam
   __shared__ uint64_t seen[kSet];
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
@@ -1927,7 +1927,7 @@ __global__ void __launch_bounds__(64) k_hou_seam(const Tube *__restrict__ tubes,
     pd = nd;
     // the first lane to meet a pair unites it; a full set just unites
     const uint64_t key = (uint64_t)nu << 32 | nd;
-    uint32_t hsh = (uint32_t)((key * 0x9E3779B97F4A7C15ULL) >> 55);
+    uint32_t hsh = (uint32_t)((key * 0x9E3779B97F4A7C15ULL) >> (64 - kSetBits));
     bool fresh = true;
     for (int t = 0; t < kSet; ++t, hsh = (hsh + 1) & (kSet - 1)) {
       const unsigned long long old =
