@@ -298,10 +298,13 @@ def test_full_size_pt_matches_oracle_pt():
     assert bytes(ost) == bytes(st2)
 
 
-def test_pt_many_temperatures():
+@pytest.mark.parametrize("nt", [220, 255])
+def test_pt_many_temperatures(nt):
     """220 temperatures: one PT call draws 219 numbers, so the generator's
-    256-slot ring wraps within the call."""
-    o, e = pair(8, 8, 2, 220, 2, bc=0, tmin=0.4, tmax=2.0,
+    256-slot ring wraps within the call.  255, the most the byte-wide
+    permutations allow: every lane stages four slots and the walk runs in
+    four chunks of pairs, the last one partial."""
+    o, e = pair(8, 8, 2, nt, 2, bc=0, tmin=0.4, tmax=2.0,
                 seeds=(61, 62, 63, 64))
     for chunk in (1, 3):
         e.steps(chunk, cluster=False)
@@ -312,6 +315,7 @@ def test_pt_many_temperatures():
 @pytest.mark.parametrize("case", [
     dict(L=8, Lz=16, nr=4, nt=16, ns=5, bc=1, tmin=0.7, tmax=1.6, seeds=(11, 12, 13, 14)),
     dict(L=8, Lz=8, nr=2, nt=220, ns=2, bc=0, tmin=0.4, tmax=2.0, seeds=(61, 62, 63, 64)),
+    dict(L=8, Lz=8, nr=2, nt=255, ns=3, bc=1, tmin=0.4, tmax=2.0, seeds=(65, 66, 67, 68)),
 ])
 def test_exact_pt_walk_matches_oracle(case, monkeypatch):
     """Every PT walk forced onto the exact rule pair by pair (the path a walk
