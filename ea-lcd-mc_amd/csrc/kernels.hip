@@ -1426,6 +1426,16 @@ struct Rows {
     const int s = hi(m);
     return (!(b & 1) && s == hi(b)) ? 0 : s;
   }
+  // rep(b, x) without a branch, from two constants of the row: b1 = b | 1
+  // and hbe = wrap_start(b, L), where the run that wraps past L-1 starts (L
+  // when none does); sites from hbe on belong to the row start, and bit 0 of
+  // b1 gives the sites before the first boundary rep 0
+  static __device__ __forceinline__ int wrap_start(RowT b, int L) {
+    return (b && !(b & 1)) ? hi(b) : L;
+  }
+  static __device__ __forceinline__ int rep_at(RowT b1, int hbe, int x) {
+    return x >= hbe ? 0 : hi(b1 & ((RowT(2) << x) - 1));
+  }
 };
 
 // Cluster labelling of a slab of `planes` full z-planes in LDS, all threads
@@ -1600,17 +1610,19 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   const RowT full = (RowT)(L == 64 ? ~0ULL : ((1ULL << L) - 1));
   for (int rr = tid; rr < nrows; rr += T) {
     const int r = spread_row(rr, nrows);
-    const RowT q = (RowT)rows[r], b = R::bounds(q, L, full);
+    const RowT q = (RowT)rows[r], b = R::bounds(q, L, full), b1 = b | 1;
+    const int hbe = R::wrap_start(b, L);
     const RowT lq = ~(q ^ (q << 1)) & full & ~RowT(1);  // q[x] == q[x-1]
     const int zl = qdiv(r, g.mL), y = r - zl * L;
     auto link = [&](int r2) {
-      const RowT q2 = (RowT)rows[r2], b2 = R::bounds(q2, L, full);
+      const RowT q2 = (RowT)rows[r2], b2 = R::bounds(q2, L, full), b21 = b2 | 1;
+      const int hbe2 = R::wrap_start(b2, L);
       const RowT l2 = ~(q2 ^ (q2 << 1)) & full & ~RowT(1);
       RowT todo = ~(q ^ q2) & full & ~(lq & l2);  // where a run overlap starts
       while (todo) {
         const int x = R::lo(todo);
         todo &= todo - 1;
-        f.unite(r * L + R::rep(b, x), r2 * L + R::rep(b2, x));
+        f.unite(r * L + R::rep_at(b1, hbe, x), r2 * L + R::rep_at(b21, hbe2, x));
       }
     };
     link(y + 1 == L ? r - (L - 1) : r + 1);
