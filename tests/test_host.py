@@ -44,6 +44,52 @@ def test_errors_are_reported_not_fatal():
     assert eatube.lib().ea_batch_set_shape(None, C.byref(sh)) == -1
 
 
+def _config(nt=8, sorted_betas=True, with_tables=True, **over):
+    """A valid configuration (L=16, Lz=16, 4 replicas, 8 betas, 2 samples,
+    PBC, 4 cluster betas) with `over` replacing fields."""
+    f = dict(L=16, Lz=16, nr=4, nt=nt, ns=2, bc=0, clone_threshold=4,
+             job_id=0)
+    f.update(over)
+    betas = np.linspace(1.6, 0.6, max(nt, 1))
+    if not sorted_betas:
+        betas = betas[::-1].copy()
+    keep = (betas, np.zeros(max(nt, 1) * 10240, dtype=np.uint8))
+    b = keep[0].ctypes.data_as(C.POINTER(C.c_double)) if with_tables else None
+    lut = keep[1].ctypes.data_as(C.POINTER(C.c_uint8)) if with_tables else None
+    c = eatube.Config(f["L"], f["Lz"], f["nr"], f["nt"], f["ns"], f["bc"],
+                      f["clone_threshold"], f["job_id"], b, lut)
+    return c, keep
+
+
+@pytest.mark.parametrize("over,msg", [
+    (dict(L=12), b"multiples of 8"),
+    (dict(Lz=20), b"multiples of 8"),
+    (dict(nr=3), b"power of two"),
+    (dict(nr=1), b"power of two"),
+    (dict(nr=32), b"power of two"),
+    (dict(nt=1), b"out of 2..255"),
+    (dict(nt=256), b"out of 2..255"),
+    (dict(ns=0), b"out of 1..31"),
+    (dict(ns=32), b"out of 1..31"),
+    (dict(bc=2), b"bc=2"),
+    (dict(clone_threshold=-1), b"clone_threshold"),
+    (dict(clone_threshold=9), b"clone_threshold"),
+    (dict(job_id=1 << 14), b"14-bit"),
+    (dict(with_tables=False), b"betas and lut are required"),
+    (dict(sorted_betas=False), b"sorted from largest to smallest"),
+])
+def test_invalid_configurations_fail_with_their_reason(over, msg):
+    """Every check ea_create makes before it looks for a device: the call
+    returns EA_E_ARG (-1) with the reason, and no engine."""
+    c, keep = _config(**over)
+    h = C.c_void_p(1)
+    rc = eatube.lib().ea_create(C.byref(c), C.byref(h))
+    assert rc == -1
+    assert h.value is None
+    assert msg in eatube.lib().ea_last_error(), eatube.lib().ea_last_error()
+    del keep
+
+
 @pytest.mark.parametrize("seed", [1, 12345, 0xDEADBEEFCAFE, 2**63 + 5])
 def test_seed_lists_and_hq(seed):
     assert np.array_equal(eatube.seeds_from_one(seed, 32),
