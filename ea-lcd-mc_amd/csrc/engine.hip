@@ -173,6 +173,11 @@ static int check_config(const ea_config *c) {
   if (!c) return fail(EA_E_ARG, "config is NULL");
   if (!valid_lattice(c->L, c->Lz))
     return fail(EA_E_ARG, "L=%d Lz=%d: both must be multiples of 8", c->L, c->Lz);
+  if (const char *d = inexact_divisor(c->L, c->Lz, c->clone_threshold > 0))
+    return fail(EA_E_ARG,
+                "L=%d Lz=%d: the kernels' index quotients by %s are not exact over this "
+                "tube's index range",
+                c->L, c->Lz, d);
   if (c->nr < 2 || c->nr > EA_MAX_REPLICAS || (c->nr & (c->nr - 1)))
     return fail(EA_E_ARG, "nr=%d: a power of two in 2..%d", c->nr, EA_MAX_REPLICAS);
   if (c->nt < 2 || c->nt > EA_MAX_BETAS)
@@ -243,15 +248,25 @@ static int stage_inputs(ea_engine *e, const ea_counter *ctr, size_t nctr, const 
   return EA_OK;
 }
 
+// Grows the TRW history to `rows` rows; the rows written so far are kept
+// (a caller may advance it0 across ea_run calls of one bin), new rows are 0.
+// On failure the old buffer stays in place.
 static int ensure_hist(ea_engine *e, int rows) {
   const Lattice &g = e->g();
   if (rows <= e->hist_rows) return EA_OK;
+  const size_t per = (size_t)g.ns * g.nr * g.nt;
   HIP_TRY(hipStreamSynchronize(e->stream));
+  uint8_t *h = nullptr;
+  RC_TRY(dalloc(&h, (size_t)rows * per));
+  hipError_t err = hipMemset(h, 0, (size_t)rows * per);
+  if (err == hipSuccess && e->t.hist)
+    err = hipMemcpy(h, e->t.hist, (size_t)e->hist_rows * per, hipMemcpyDeviceToDevice);
+  if (err != hipSuccess) {
+    (void)hipFree(h);
+    return fail(EA_E_HIP, "growing the history: %s", hipGetErrorString(err));
+  }
   if (e->t.hist) HIP_TRY(hipFree(e->t.hist));
-  e->t.hist = nullptr;
-  e->hist_rows = 0;
-  RC_TRY(dalloc(&e->t.hist, (size_t)rows * g.ns * g.nr * g.nt));
-  HIP_TRY(hipMemset(e->t.hist, 0, (size_t)rows * g.ns * g.nr * g.nt));
+  e->t.hist = h;
   e->hist_rows = rows;
   return EA_OK;
 }
@@ -622,20 +637,30 @@ static int ensure_energy(ea_engine *e) {
   return EA_OK;
 }
 
-extern "C" int ea_sweep(ea_engine *e, int nsweeps, const ea_counter *blocks) {
-  if (!e || (nsweeps > 0 && !blocks)) return fail(EA_E_ARG, "NULL argument");
-  if (nsweeps < 0) return fail(EA_E_ARG, "nsweeps < 0");
-  if (nsweeps == 0) return EA_OK;
-  RC_TRY(stage_inputs(e, blocks, (size_t)nsweeps * 2 * e->g().nr, nullptr, 0));
-  RC_TRY(begin_calls(e, 1, 2 * nsweeps, 0, 0, 1));
+// Enqueues the 2*nsweeps half-sweeps of one call whose counter blocks are
+// already on the device at `ctr` (timed by ea_set_timing when `timed`).
+static int enqueue_sweeps(ea_engine *e, int nsweeps, const ea_counter *ctr, bool timed) {
+  const ea_counter *own = e->t.ctr;
+  e->t.ctr = ctr;  // the device copy of the descriptor carries it
+  const int rc = begin_calls(e, 1, 2 * nsweeps, 0, 0, 1);
+  e->t.ctr = own;
+  RC_TRY(rc);
   for (int half = 0; half < 2 * nsweeps; ++half) {
     Phase ph{half, half == 2 * nsweeps - 1, 0, 0, 0};
-    Timed t(e, EA_K_SWEEP);
+    Timed t(timed ? e : nullptr, EA_K_SWEEP);
     HIP_TRY(launch_sweep(e->dt, e->plan, ph, e->stream));
   }
   e->part_pending = true;
   e->ener_valid = false;
   return EA_OK;
+}
+
+extern "C" int ea_sweep(ea_engine *e, int nsweeps, const ea_counter *blocks) {
+  if (!e || (nsweeps > 0 && !blocks)) return fail(EA_E_ARG, "NULL argument");
+  if (nsweeps < 0) return fail(EA_E_ARG, "nsweeps < 0");
+  if (nsweeps == 0) return EA_OK;
+  RC_TRY(stage_inputs(e, blocks, (size_t)nsweeps * 2 * e->g().nr, nullptr, 0));
+  return enqueue_sweeps(e, nsweeps, e->t.ctr, true);
 }
 
 extern "C" int ea_energy(ea_engine *e, int32_t *out) {
@@ -741,7 +766,12 @@ extern "C" int ea_set_shape(ea_engine *e, const ea_shape *shape) {
   if (!shape_fits(e->g(), to_shape(shape)))
     return fail(EA_E_ARG, "sweep_kind %d: this tube's tile (256 + L*L/16 half sites) exceeds 768",
                 shape->sweep_kind);
-  if (e->part_pending) RC_TRY(ensure_energy(e));
+  if (e->part_pending) {
+    // the resolve launch exits unless a call is in flight (a previous
+    // ea_run with n_it = 0 leaves none)
+    RC_TRY(begin_calls(e, 1, 0, 0, 0, 1));
+    RC_TRY(ensure_energy(e));
+  }
   e->shape = to_shape(shape);
   e->t.nbx = sweep_partials_x(e->g(), e->shape);
   RC_TRY(upload_tube(e));

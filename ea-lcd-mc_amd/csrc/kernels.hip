@@ -111,8 +111,10 @@ __device__ __forceinline__ uint32_t rnd_word(uint32_t x, uint32_t y, uint32_t z,
   return (bb << rot) | (bb >> ((32 - rot) & 31));
 }
 
-// Quotient i / d from m = ceil(2^32 / d) (m = 0 for d = 1): exact for
-// i < 2^26 and d <= 64, one multiply instead of a division sequence.
+// Quotient i / d from m = ceil(2^32 / d) (m = 0 for d = 1), one multiply
+// instead of a division sequence: exact while i * (m d - 2^32) < 2^32, which
+// holds for every divisor and index range of the tubes the engine accepts
+// (inexact_divisor refuses the others at creation).
 __device__ __forceinline__ uint32_t qdiv(uint32_t i, uint32_t m) {
   return m ? __umulhi(i, m) : i;
 }

@@ -53,6 +53,35 @@ inline void set_divisors(Lattice &g) {
   g.mhs = divisor_magic(g.hs);
 }
 
+// Whether the kernels' quotient by multiply, hi32(i * m) with
+// m = ceil(2^32 / d) = (2^32 + e) / d, equals i / d for every i <= imax:
+// i m / 2^32 = i / d + i e / (d 2^32), and the fraction of i / d is at most
+// (d - 1) / d, so the quotient is exact when imax * e < 2^32.
+inline bool qdiv_exact(uint32_t d, uint64_t imax) {
+  if (d <= 1) return true;  // m = 0: the index itself
+  const uint64_t e = (uint64_t)divisor_magic(d) * d - (1ULL << 32);
+  return imax * e < (1ULL << 32);
+}
+
+// The first divisor of an L x L x Lz tube whose quotients the kernels would
+// get wrong over that divisor's index range, or NULL when all are exact.
+// Every kernel divides by L/8 the remainder within a plane of half sites
+// (< L*L/32) and by L*L/32 the half-site index (< V/128); the Houdayer
+// kernels (`cluster`) also divide site and row indices (< V) by L, the y
+// coordinate (< L) by L/4 and the z coordinate (< Lz) by Lz/4.
+inline const char *inexact_divisor(int L, int Lz, bool cluster) {
+  const uint64_t V = (uint64_t)L * L * Lz;
+  const uint32_t ML = L / 4, hs = ML * ML / 2;
+  if (!qdiv_exact(ML / 2, hs - 1)) return "L/8";
+  if (!qdiv_exact(hs, V / 128 - 1)) return "L*L/32";
+  if (cluster) {
+    if (!qdiv_exact(L, V - 1)) return "L";
+    if (!qdiv_exact(ML, L - 1)) return "L/4";
+    if (!qdiv_exact(Lz / 4, Lz - 1)) return "Lz/4";
+  }
+  return nullptr;
+}
+
 // Everything the kernels need about one tube: geometry, device state and the
 // inputs of the calls in flight.  Per-call inputs are indexed by the tube's
 // device step counter, so one captured step can be replayed: step k reads

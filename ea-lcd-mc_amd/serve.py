@@ -41,7 +41,8 @@ class Request:
     ns: int = 1
     tmin: float = 0.6
     tmax: float = 1.6
-    cluster: bool = False
+    cluster: bool = True  # the reference program's step has the Houdayer
+                          # move unless built with NO_CLUSTER (main.cu:182-184)
 
     def seeds(self):
         s = int(self.seed) * 4
@@ -68,10 +69,11 @@ class Result:
 
 
 def poisson_requests(n, rate, seed=1, sizes=(8, 16, 24, 32), nts=(16, 24, 32),
-                     sweeps=(1000, 100000), lz_max=16):
+                     sweeps=(1000, 100000), lz_max=16, cluster=True):
     """n requests at Poisson arrivals of `rate` per second: L from `sizes`,
     Lz a multiple of 8 in [L, lz_max*L], PBC or OBC, NT from `nts`, sweeps
-    log-uniform in [sweeps[0], sweeps[1]]."""
+    log-uniform in [sweeps[0], sweeps[1]]; each step has the Houdayer move
+    when `cluster`."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -83,7 +85,7 @@ def poisson_requests(n, rate, seed=1, sizes=(8, 16, 24, 32), nts=(16, 24, 32),
                                             math.log(sweeps[1])))))
         out.append(Request(rid=i, seed=1 + 7919 * i, L=L, Lz=Lz,
                            bc=int(rng.integers(0, 2)), nt=int(rng.choice(nts)),
-                           sweeps=sw, arrival=t))
+                           sweeps=sw, arrival=t, cluster=bool(cluster)))
     return out
 
 
@@ -152,100 +154,126 @@ class Server:
 
     def run(self, requests):
         """Serve `requests` (arrival times relative to now); returns the
-        results in request order."""
+        results in request order.  A request whose footprint exceeds the
+        budget is refused (ValueError) before anything starts; if serving
+        fails later, the running engines, the batch and the worker pool are
+        released before the error propagates."""
+        need_of = {}
+        for r in requests:
+            need_of[r.rid] = self.footprint_fn(r, self.luts)
+            if need_of[r.rid] > self.budget:
+                raise ValueError("request %d needs %d bytes, more than the "
+                                 "budget" % (r.rid, need_of[r.rid]))
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         pool = ThreadPoolExecutor(self.workers) if self.workers > 0 else None
+        running = {}  # rid -> [request, sim, steps left, bytes, admitted]
+        batch = None
+        try:
+            results, batch = self._serve(pending, need_of, pool, running)
+        finally:
+            if batch is not None:
+                batch.close()
+            for ent in running.values():
+                ent[1].engine.close()
+            if pool is not None:
+                pool.shutdown(wait=True, cancel_futures=True)
+        return [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+
+    def _serve(self, pending, need_of, pool, running):
         futures = {}  # rid -> preparation in flight
         t0 = self.clock()
-        running = {}  # rid -> [request, sim, steps left, bytes, admitted]
         used = 0
         results = {}
         batch, members = None, ()
-        while pending or running:
-            now = self.clock() - t0
-            if pool is not None:  # arrived requests start preparing at once
-                for r in pending:
-                    if r.arrival > now:
+        try:
+            while pending or running:
+                now = self.clock() - t0
+                if pool is not None:  # arrived requests start preparing at once
+                    for r in pending:
+                        if r.arrival > now:
+                            break
+                        if r.rid not in futures:
+                            futures[r.rid] = pool.submit(self.prepare_fn, r,
+                                                         self.luts)
+                # admission: arrived (and prepared) requests, in arrival
+                # order, while they fit
+                k = 0
+                while k < len(pending) and pending[k].arrival <= now and \
+                        len(running) < self.max_tubes:
+                    r = pending[k]
+                    if pool is not None and not futures[r.rid].done():
+                        k += 1
+                        continue
+                    need = need_of[r.rid]
+                    if used + need > self.budget:
+                        self.stats["deferred"] += 1
                         break
-                    if r.rid not in futures:
-                        futures[r.rid] = pool.submit(self.prepare_fn, r, self.luts)
-            # admission: arrived (and prepared) requests, in arrival order,
-            # while they fit
-            k = 0
-            while k < len(pending) and pending[k].arrival <= now and \
-                    len(running) < self.max_tubes:
-                r = pending[k]
-                if pool is not None and not futures[r.rid].done():
-                    k += 1
+                    pending.pop(k)
+                    ts = self.clock()
+                    if pool is None:
+                        sim = self.start_fn(r, self.luts)
+                    else:
+                        sim = self.start_fn(r, self.luts,
+                                            futures.pop(r.rid).result())
+                    running[r.rid] = [r, sim, r.sweeps, need, self.clock() - t0]
+                    self.stats["t_start"] += self.clock() - ts
+                    used += need
+                self.stats["max_running"] = max(self.stats["max_running"],
+                                                len(running))
+                self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
+                if not running:
+                    if pending and pending[0].arrival > self.clock() - t0:
+                        time.sleep(max(0.0, pending[0].arrival -
+                                       (self.clock() - t0)))
+                    else:
+                        time.sleep(0.0005)  # arrived requests still preparing
                     continue
-                need = self.footprint_fn(r, self.luts)
-                if need > self.budget:
-                    raise ValueError("request %d needs %d bytes, more than the "
-                                     "budget" % (r.rid, need))
-                if used + need > self.budget:
-                    self.stats["deferred"] += 1
-                    break
-                pending.pop(k)
+                # the running requests, ordered by size, are the batch's tubes
+                order = tuple(sorted(running, key=lambda k: (
+                    running[k][0].L * running[k][0].L * running[k][0].Lz, k)))
                 ts = self.clock()
-                if pool is None:
-                    sim = self.start_fn(r, self.luts)
-                else:
-                    sim = self.start_fn(r, self.luts, futures.pop(r.rid).result())
-                running[r.rid] = [r, sim, r.sweeps, need, self.clock() - t0]
-                self.stats["t_start"] += self.clock() - ts
-                used += need
-            self.stats["max_running"] = max(self.stats["max_running"], len(running))
-            self.stats["max_bytes"] = max(self.stats["max_bytes"], used)
-            if not running:
-                if pending and pending[0].arrival > self.clock() - t0:
-                    time.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
-                else:
-                    time.sleep(0.0005)  # arrived requests still preparing
-                continue
-            # the running requests, ordered by size, are the batch's tubes
-            order = tuple(sorted(running, key=lambda k: (
-                running[k][0].L * running[k][0].L * running[k][0].Lz, k)))
-            ts = self.clock()
-            if order != members:
-                if batch is not None:
-                    batch.close()
-                batch = self.batch_fn([running[k][1].engine for k in order])
-                members = order
-                self.stats["batches_built"] += 1
-            tb = self.clock()
-            scheds = []
-            for k in order:
-                r, sim, left = running[k][:3]
-                scheds.append(sim.schedule(min(self.chunk, left),
-                                           cluster=r.cluster))
-            tc = self.clock()
-            batch.run(scheds)
-            self.stats["chunks"] += 1
-            done = self.clock() - t0
-            self.stats["t_batch"] += tb - ts
-            self.stats["t_schedule"] += tc - tb
-            self.stats["t_run"] += self.clock() - tc
-            ts = self.clock()
-            for k, sch in zip(order, scheds):
-                running[k][2] -= sch.n_it
-                if running[k][2] == 0:
-                    r, sim, _, need, adm = running.pop(k)
-                    res = Result(r.rid, r.arrival, adm, done)
-                    if self.keep:
-                        es, _ = sim.engine.read_measures()
-                        res.spins = sim.spins()
-                        res.perms = sim.engine.download_perms()
-                        res.esum = es
-                        res.energies = sim.engine.energy()
-                    sim.engine.close()
-                    used -= need
-                    results[r.rid] = res
-            self.stats["t_finish"] += self.clock() - ts
-        if batch is not None:
-            batch.close()
-        if pool is not None:
-            pool.shutdown()
-        return [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+                if order != members:
+                    if batch is not None:
+                        batch.close()
+                        batch = None
+                    batch = self.batch_fn([running[k][1].engine for k in order])
+                    members = order
+                    self.stats["batches_built"] += 1
+                tb = self.clock()
+                scheds = []
+                for k in order:
+                    r, sim, left = running[k][:3]
+                    scheds.append(sim.schedule(min(self.chunk, left),
+                                               cluster=r.cluster))
+                tc = self.clock()
+                batch.run(scheds)
+                self.stats["chunks"] += 1
+                done = self.clock() - t0
+                self.stats["t_batch"] += tb - ts
+                self.stats["t_schedule"] += tc - tb
+                self.stats["t_run"] += self.clock() - tc
+                ts = self.clock()
+                for k, sch in zip(order, scheds):
+                    running[k][2] -= sch.n_it
+                    if running[k][2] == 0:
+                        r, sim, _, need, adm = running[k]
+                        res = Result(r.rid, r.arrival, adm, done)
+                        if self.keep:
+                            es, _ = sim.engine.read_measures()
+                            res.spins = sim.spins()
+                            res.perms = sim.engine.download_perms()
+                            res.esum = es
+                            res.energies = sim.engine.energy()
+                        del running[k]
+                        sim.engine.close()
+                        used -= need
+                        results[r.rid] = res
+                self.stats["t_finish"] += self.clock() - ts
+        except BaseException:
+            if batch is not None:
+                batch.close()
+            raise
+        return results, batch
 
 
 def summary(requests, results, wall):

@@ -400,3 +400,27 @@ def test_footprint_matches_allocation(L, Lz, ns):
     # runtime buffers
     assert want * 0.95 - (16 << 20) <= used <= want * 1.05 + (16 << 20)
     sim.engine.close()
+
+
+def test_shape_change_after_an_empty_run_sums_pending_energies():
+    """ea_sweep leaves partial energies pending; ea_run with n_it = 0 leaves
+    no call in flight; ea_set_shape must still sum those partials under the
+    old layout before changing it, so that the next PT decides from the
+    energies of the swept configuration (perms, PT generators and
+    statistics against the oracle's sweep, energy and PT)."""
+    o, e = pair(16, 16, 4, 8, 3, bc=1, seeds=(41, 42, 43, 44))
+    e.engine.sweep(e.counters(4))  # two sweeps
+    o.sweeps(2)
+    sch = eatube.Schedule(0, 1, 1, 1, 0, 0, 1, None, None, 0, 0)
+    off = C.c_uint64(0)
+    eatube.check(eatube.lib().ea_run(e.engine.h, C.byref(sch), C.byref(off)))
+    e.engine.set_shape(32, 512, 0, 1)
+    e.engine.pt()
+    o.energy()
+    o.pt()
+    assert np.array_equal(e.engine.download_perms().ravel(), o.field("wctb"))
+    st, acc, att, raw = e.engine.download_pt_state()
+    assert bytes(st) == bytes(o.field("pt"))
+    assert np.array_equal(acc.ravel(), o.field("acc"))
+    assert np.array_equal(raw.ravel(), o.field("raw"))
+    assert np.array_equal(e.engine.energy().ravel(), o.field("ener"))

@@ -90,6 +90,39 @@ def test_invalid_configurations_fail_with_their_reason(over, msg):
     del keep
 
 
+def test_inexact_index_quotients_are_refused():
+    """L=112, Lz=144168 passes the size checks, but the half-site quotient
+    by L*L/32 = 392 taken by multiply is wrong at h = 14128463 (36042
+    instead of 36041): creation and the footprint query refuse the tube
+    instead of letting the sweep update the wrong neighbours."""
+    c, keep = _config(L=112, Lz=144168, clone_threshold=0)
+    h = C.c_void_p(1)
+    assert eatube.lib().ea_create(C.byref(c), C.byref(h)) == -1
+    assert h.value is None
+    msg = eatube.lib().ea_last_error()
+    assert b"not exact" in msg and b"L*L/32" in msg, msg
+    out = C.c_uint64()
+    assert eatube.lib().ea_footprint(C.byref(c), C.byref(out)) == -1
+    assert b"not exact" in eatube.lib().ea_last_error()
+    del keep
+
+
+def workload_lattices():
+    """Every (L, Lz) of the plain, OBC, batched, large and serving
+    workloads: 8x16, 16x64, L in {8, 16, 24} with Lz in {L, 4L, 8L},
+    48x384, and L in {8, 16, 24, 32} with Lz a multiple of 8 in [L, 16L]."""
+    out = {(8, 16), (16, 64), (48, 384)}
+    out |= {(L, k * L) for L in (8, 16, 24) for k in (1, 4, 8)}
+    out |= {(L, Lz) for L in (8, 16, 24, 32) for Lz in range(L, 16 * L + 1, 8)}
+    return sorted(out)
+
+
+def test_every_workload_lattice_passes_the_quotient_check():
+    betas = np.array([1.6, 0.6])
+    for L, Lz in workload_lattices():
+        assert eatube.footprint(L, Lz, 4, 2, 1, 0, betas) > 0, (L, Lz)
+
+
 @pytest.mark.parametrize("seed", [1, 12345, 0xDEADBEEFCAFE, 2**63 + 5])
 def test_seed_lists_and_hq(seed):
     assert np.array_equal(eatube.seeds_from_one(seed, 32),

@@ -101,6 +101,43 @@ def test_oversized_request_is_refused():
         srv.run([req(0, 16, 10, 0.0)])
 
 
+def test_oversized_request_is_refused_before_anything_starts():
+    srv, log, _ = harness(budget=20000, chunk=4)
+    with pytest.raises(ValueError):
+        srv.run([req(0, 8, 10, 0.0), req(1, 16, 10, 0.5), req(2, 24, 10, 1.0)])
+    assert log == []  # request 2 (24000 bytes) was refused up front
+
+
+def test_failure_releases_running_engines():
+    """A start that fails while other requests run: the error propagates,
+    and the running requests' engines and the batch are closed."""
+    srv, log, sims = harness(budget=10 ** 9, chunk=5)
+    closed_batches = []
+    inner_start = srv.start_fn
+
+    def start(r, luts):
+        if r.rid == 2:
+            raise RuntimeError("device start failed")
+        return inner_start(r, luts)
+
+    class Batch:
+        def __init__(self, engines):
+            self.n = len(engines)
+
+        def run(self, scheds):
+            srv.clock.t += 0.01 * max(s.n_it for s in scheds)
+
+        def close(self):
+            closed_batches.append(self.n)
+
+    srv.start_fn, srv.batch_fn = start, Batch
+    reqs = [req(0, 8, 100, 0.0), req(1, 16, 100, 0.0), req(2, 8, 10, 0.05)]
+    with pytest.raises(RuntimeError):
+        srv.run(reqs)
+    assert sims[0].engine.closed and sims[1].engine.closed
+    assert closed_batches == [2]
+
+
 def test_poisson_requests_shapes():
     rs = serve.poisson_requests(300, rate=50.0, seed=3)
     assert all(r.L in (8, 16, 24, 32) and r.Lz % 8 == 0 and
