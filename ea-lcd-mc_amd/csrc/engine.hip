@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -80,6 +81,7 @@ struct ea_engine {
   uint32_t job = 0;
   std::vector<double> betas;
   size_t ctr_cap = 0, pairs_cap = 0;
+  ea_counter *noalloc_ctr = nullptr;  // this device's `scratch` address (not owned)
   int hist_rows = 0;
   int keep_on = 0;
   uint32_t *keep_buf = nullptr;
@@ -147,6 +149,29 @@ struct ea_batch {
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+// Counter blocks of ea_sweep_noalloc: one array per GPU, shared by every
+// engine there and held until the process ends.  Host code uses only the
+// per-device address hipGetSymbolAddress returns (scratch_address).
+static __device__ __align__(256) ea_counter scratch[EA_NOALLOC_COUNTERS];
+
+// Device address of `scratch` on `dev`, looked up once per device with that
+// device current (ea_create calls it for the device it binds).
+static int scratch_address(int dev, ea_counter **out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static ea_counter *addr[kMaxDevices] = {};
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return fail(EA_E_NODEV, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    addr[dev] = static_cast<ea_counter *>(p);
+  }
+  *out = addr[dev];
+  return EA_OK;
+}
 
 extern "C" const char *ea_last_error(void) { return g_err.c_str(); }
 
@@ -297,6 +322,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
 
   auto e = std::make_unique<ea_engine>();
   e->device = dev;
+  RC_TRY(scratch_address(dev, &e->noalloc_ctr));
   Tube &t = e->t;
   Lattice &g = t.g;
   g.L = cfg->L;
@@ -661,6 +687,23 @@ extern "C" int ea_sweep(ea_engine *e, int nsweeps, const ea_counter *blocks) {
   if (nsweeps == 0) return EA_OK;
   RC_TRY(stage_inputs(e, blocks, (size_t)nsweeps * 2 * e->g().nr, nullptr, 0));
   return enqueue_sweeps(e, nsweeps, e->t.ctr, true);
+}
+
+extern "C" int ea_sweep_noalloc(ea_engine *e, int nsweeps, const ea_counter *blocks) {
+  if (!e || (nsweeps > 0 && !blocks)) return fail(EA_E_ARG, "NULL argument");
+  if (nsweeps < 0) return fail(EA_E_ARG, "nsweeps < 0");
+  const size_t nctr = (size_t)nsweeps * 2 * e->g().nr;
+  if (nctr > EA_NOALLOC_COUNTERS)
+    return fail(EA_E_ARG,
+                "ea_sweep_noalloc: %d sweeps of %d replicas need %zu counter blocks, more than "
+                "the %d of the shared buffer",
+                nsweeps, e->g().nr, nctr, EA_NOALLOC_COUNTERS);
+  if (nsweeps == 0) return EA_OK;
+  if (!e->noalloc_ctr) return fail(EA_E_STATE, "no counter buffer on this engine's device");
+  // the engine's previous kernels may still read the buffer
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HIP_TRY(hipMemcpy(e->noalloc_ctr, blocks, sizeof(ea_counter) * nctr, hipMemcpyHostToDevice));
+  return enqueue_sweeps(e, nsweeps, e->noalloc_ctr, false);
 }
 
 extern "C" int ea_energy(ea_engine *e, int32_t *out) {

@@ -74,6 +74,7 @@ EXPORTS = [
     "ea_read_timing", "ea_batch_create", "ea_batch_destroy", "ea_batch_run",
     "ea_footprint", "ea_select_device", "ea_pack_lattice", "ea_conf_bytes",
     "ea_conf_encode", "ea_conf_decode", "ea_set_shape", "ea_batch_set_shape",
+    "ea_sweep_noalloc",
 ]
 KINDS = ("sweep", "energy", "pt", "houdayer", "measure")
 
@@ -110,6 +111,7 @@ def lib():
         "ea_download_pt_state": (C.c_int, [vp, p(HQ), p(C.c_double),
                                            p(C.c_double), p(C.c_double)]),
         "ea_sweep": (C.c_int, [vp, C.c_int, p(Counter)]),
+        "ea_sweep_noalloc": (C.c_int, [vp, C.c_int, p(Counter)]),
         "ea_energy": (C.c_int, [vp, p(i32)]),
         "ea_overlap": (C.c_int, [vp, p(i32)]),
         "ea_pt": (C.c_int, [vp]),
@@ -324,6 +326,13 @@ class Engine:
             blocks, C.Array) else blocks
         nsw = len(b) // (2 * self.nr)
         check(lib().ea_sweep(self.h, nsw, b))
+
+    def sweep_noalloc(self, blocks):
+        """ea_sweep through the per-GPU counter buffer (no allocation)."""
+        b = (Counter * len(blocks))(*blocks) if not isinstance(
+            blocks, C.Array) else blocks
+        nsw = len(b) // (2 * self.nr)
+        check(lib().ea_sweep_noalloc(self.h, nsw, b))
 
     def energy(self):
         out = np.zeros((self.ns, self.nr, self.nt), np.int32)

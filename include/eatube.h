@@ -153,6 +153,19 @@ int ea_download_pt_state(ea_engine *e, ea_hq_state *st, double *acc,
  * never stored. */
 int ea_sweep(ea_engine *e, int nsweeps, const ea_counter *blocks);
 
+/* ea_sweep that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * counter blocks go through one static device array per GPU of
+ * EA_NOALLOC_COUNTERS blocks (64 KiB), so nsweeps * 2 * nr must not exceed
+ * it (else EA_E_ARG, nothing done).  Every engine on a GPU shares that
+ * array, and a call's kernels read it until they finish: calls on one GPU
+ * whose sweeps overlap (two engines' streams) race, so wait for the
+ * previous call (ea_sync) first.  Results equal ea_sweep's; its launches
+ * are not bracketed by ea_set_timing's events. */
+#define EA_NOALLOC_COUNTERS 4096
+int ea_sweep_noalloc(ea_engine *e, int nsweeps, const ea_counter *blocks);
+
 /* Measure_Energy (main.cu:531-554): energies [ns][nr][clone] into out (may
  * be NULL: the energies stay on the device for ea_pt). */
 int ea_energy(ea_engine *e, int32_t *out);
