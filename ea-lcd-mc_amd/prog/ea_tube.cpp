@@ -112,11 +112,28 @@ void check(int rc, const char *what) {
 
 // ---------------------------------------------------------------- signals
 
-void stop_run(int signum) {  // io.c:906-918
+// The reference's handler writes the error file and exits from inside the
+// signal handler (io.c:906-918).  With HIP work in flight that can deadlock
+// in the runtime's teardown, so here the handler only records the signal,
+// and the main thread acts on it (stop_run) at the points where the engine
+// is idle: after each bin's GPU work has been waited for, before its files
+// are written, and between bins.  A signal that arrives while a bin runs
+// thus ends the run when that bin's GPU work is done, with the same result
+// as the reference: no files of that bin, the error file, no `running`,
+// exit code 1.
+volatile sig_atomic_t g_signal = 0;
+
+void note_signal(int signum) { g_signal = signum; }
+
+[[noreturn]] void stop_run(int signum) {  // io.c:906-918
   if (signum == SIGTERM) create_error("Received SIGTERM\n");
   if (signum == SIGINT) create_error("Received SIGINT\n");
   if (signum == SIGUSR1) create_error("Received SIGUSR1\n");
   print_and_exit("Aborting programma (SIGNAL=%d)\n", signum);
+}
+
+void check_signal() {
+  if (g_signal) stop_run(g_signal);
 }
 
 void ignore_sigterm() {
@@ -126,9 +143,9 @@ void ignore_sigterm() {
 }
 
 void handle_sigterm() {
-  signal(SIGTERM, stop_run);
-  signal(SIGINT, stop_run);
-  signal(SIGUSR1, stop_run);
+  signal(SIGTERM, note_signal);
+  signal(SIGINT, note_signal);
+  signal(SIGUSR1, note_signal);
 }
 
 // ------------------------------------------------------------ run control
@@ -736,15 +753,11 @@ void write_history(int ib, const std::vector<uint8_t> &hist) {  // io.c:353-386
       if ((size_t)actual < predicted) {
         create_error("There are less data than expected in history_b.dat\n");
         print_and_exit("Problems writing history\n");
-      } else if ((size_t)actual > predicted) {
-        // append mode writes at the end whatever the position: keep the
-        // first `predicted` bytes and append after them
-        fclose(f);
-        if (truncate(name.c_str(), (off_t)predicted) != 0 || !(f = fopen(name.c_str(), "a"))) {
-          create_error("Problems opening history_b.dat\n");
-          print_and_exit("Problems writing history\n");
-        }
       }
+      // A longer file (rows of a bin that a resumed run repeats) keeps its
+      // bytes: the reference seeks to the predicted size, but a stream
+      // opened for appending writes at the end wherever it is positioned, so
+      // its rows follow the stale ones; the same bytes are written here.
       for (int k = 0; k < maxit_TRW; ++k)
         fwrite(&hist[(((size_t)k * nbits + i) * nr + r) * nt], 1, (size_t)nt, f);
       fclose(f);
@@ -930,6 +943,7 @@ int main(int argc, char **argv) {
   create_running();
 
   init_all();
+  check_signal();
   if (data.flag == 0) {  // backup (io.c:824-838)
     read_conf();
     print_data(stdout, &data);
@@ -960,6 +974,7 @@ int main(int argc, char **argv) {
   measure_time();
   int ibin;
   for (ibin = data.itcut; ibin < data.nbin; ibin++) {
+    check_signal();
     renew_running();
     ea_fill_counters(S.prc.data(), S.xo.data(), nr, iter, halves, (uint32_t)sample,
                      blocks.data());
@@ -976,6 +991,7 @@ int main(int argc, char **argv) {
     check(ea_download_pt_state(S.e, S.pt.data(), S.acc.data(), S.att.data(), S.raw.data()),
           "ea_download_pt_state");
     check(ea_download_perms(S.e, S.wctb.data()), "ea_download_perms");
+    check_signal();  // the bin's GPU work is done
     measure_time();
     ignore_sigterm();
     write_history(ibin, hist);
@@ -986,6 +1002,7 @@ int main(int argc, char **argv) {
     handle_sigterm();
     if (check_time(max_time)) break;
   }
+  check_signal();
   ignore_sigterm();
   delete_running();
   printf("Simulation I=%d, nbits=%d finished at iteration %d\n", sample, nbits, ibin);

@@ -1144,22 +1144,37 @@ void orc_sim_houdayer(orc_sim *s) {
   }
 }
 
-void orc_sim_steps(orc_sim *s, int nsteps, int ptfr, int houfr, int cluster) {
-  for (int st = 0; st < nsteps; st++) {
-    for (int ih = 0; ih < houfr; ih++) {
-      orc_sim_sweeps(s, ptfr);
-      orc_sim_energy(s);
-      orc_sim_pt(s);
+/* n_it iterations of main()'s `it` loop, main.cu:149-228: mesfr x { houfr x
+ * { ptfr sweeps, Measure_Energy, PT }, Houdayer (NO_CLUSTER: none) }, then
+ * Measure_Energy, the average_Energy sums (:209-218) and the TRW history
+ * (:221-228): iteration it0 + k with (it0 + k) % frec == frec - 1 writes
+ * which_beta_this_clon into row (it0 + k) / frec of hist[rows][ns][nr][nt]
+ * (rows past hist_rows are dropped; hist may be NULL). */
+void orc_sim_run(orc_sim *s, int n_it, int mesfr, int houfr, int ptfr,
+                 int cluster, int it0, int frec, uint8_t *hist, int hist_rows) {
+  const int nr = s->c.nr, nt = s->c.nt, ns = s->c.ns;
+  for (int k = 0; k < n_it; k++) {
+    for (int im = 0; im < mesfr; im++) {
+      for (int ih = 0; ih < houfr; ih++) {
+        orc_sim_sweeps(s, ptfr);
+        orc_sim_energy(s);
+        orc_sim_pt(s);
+      }
+      if (cluster) orc_sim_houdayer(s);
     }
-    if (cluster) orc_sim_houdayer(s);
     orc_sim_energy(s);
-    /* average_Energy accumulation, main.cu:209-218 */
-    const int nr = s->c.nr, nt = s->c.nt;
-    for (int k = 0; k < s->c.ns * nr; k++)
+    for (int c = 0; c < ns * nr; c++)
       for (int b = 0; b < nt; b++)
-        s->esum[(size_t)k * nt + b] +=
-            (double)s->ener[(size_t)k * nt + s->wctb[(size_t)k * nt + b]];
+        s->esum[(size_t)c * nt + b] +=
+            (double)s->ener[(size_t)c * nt + s->wctb[(size_t)c * nt + b]];
+    const int it = it0 + k;
+    if (hist && it % frec == frec - 1 && it / frec < hist_rows)
+      memcpy(hist + (size_t)(it / frec) * ns * nr * nt, s->wbtc, (size_t)ns * nr * nt);
   }
+}
+
+void orc_sim_steps(orc_sim *s, int nsteps, int ptfr, int houfr, int cluster) {
+  orc_sim_run(s, nsteps, 1, houfr, ptfr, cluster, 0, 1, NULL, 0);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -38,6 +38,15 @@
  *                                LUT_for_PRNG_nbits12_NB40.bin
  *   orc_overlap ............... pinned: == sum of s1*s2 over the unpacked
  *                                lattices (orc_unpack_spins, itself pinned)
+ *   orc_sim_run ............... loop nesting and TRW rows restated from
+ *                                main.cu:149-228, built from the pinned parts
+ *                                (sweeps, energy, PT, Houdayer); the counts the
+ *                                nesting fixes per `it` (half-sweeps, PT calls,
+ *                                Houdayer calls and their cluster draws, one
+ *                                energy sum, rows every frecTRW) are checked;
+ *                                the files made from its sums and rows are
+ *                                checked against the bytes of the reference's
+ *                                write_measures / write_history
  */
 #ifndef EAORACLE_H
 #define EAORACLE_H
@@ -210,8 +219,14 @@ void orc_sim_pt(orc_sim *s);
 void orc_sim_houdayer(orc_sim *s);
 /* Site overlaps [ns][pair r1 < r2][beta] of replicas at each beta. */
 void orc_overlap(const orc_sim *s, int32_t *out);
-/* nsteps of: houfr x (ptfr sweeps, energy, PT), then Houdayer (if on),
- * then energy -- the body of main.cu:149-218 for one `it` with mesfr=1 */
+/* n_it iterations of main()'s `it` loop (main.cu:149-228): mesfr x
+ * { houfr x (ptfr sweeps, energy, PT), Houdayer (if cluster) }, energy, the
+ * per-beta energy sums and the TRW history rows which_beta_this_clon
+ * hist[(it0+k)/frec][ns][nr][nt] for (it0+k) % frec == frec-1 (rows beyond
+ * hist_rows dropped; hist may be NULL). */
+void orc_sim_run(orc_sim *s, int n_it, int mesfr, int houfr, int ptfr,
+                 int cluster, int it0, int frec, uint8_t *hist, int hist_rows);
+/* orc_sim_run with mesfr = 1 and no history */
 void orc_sim_steps(orc_sim *s, int nsteps, int ptfr, int houfr, int cluster);
 /* Threads of the simulation loops (default 1); results do not depend on it. */
 void orc_set_threads(int n);

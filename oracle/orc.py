@@ -109,6 +109,7 @@ def lib():
             "orc_sim_houdayer": (None, [vp]),
             "orc_overlap": (None, [vp, C.POINTER(C.c_int32)]),
             "orc_sim_steps": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "orc_sim_run": (None, [vp] + [C.c_int] * 7 + [u8p, C.c_int]),
             "orc_set_threads": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():
@@ -317,6 +318,15 @@ class Sim:
 
     def steps(self, nsteps, ptfr=1, houfr=1, cluster=True):
         lib().orc_sim_steps(self.h, nsteps, ptfr, houfr, int(cluster))
+
+    def run(self, n_it, ptfr=1, houfr=1, mesfr=1, cluster=True, it0=0,
+            frec_trw=1, rows=0):
+        """n_it iterations of the `it` loop (orc_sim_run); returns the TRW
+        history rows [rows][ns][nr][nt] it wrote (zero where none)."""
+        hist = np.zeros((max(rows, 1), self.ns, self.nr, self.nt), np.uint8)
+        lib().orc_sim_run(self.h, n_it, mesfr, houfr, ptfr, int(cluster), it0,
+                          frec_trw, ptr(hist, C.c_uint8), rows)
+        return hist[:rows]
 
     def spins(self):
         """spin words as [ns][2][nt][n][nr]"""

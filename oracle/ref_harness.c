@@ -262,6 +262,61 @@ int main(int argc, char **argv) {
     copy_file(path, "conf_last");
   }
 
+  /* measurement files (io.c:307-397): evol.txt and aceptancePT.txt of two
+   * bins, with the PT statistics of the steps above and energy averages made
+   * up of the initial energies; history_b.dat of bins 0 and 1 and then of
+   * bin 1 again, as a run resumed from the conf of bin 0 writes it after
+   * the history of its lost bin 1 reached the file */
+  {
+    maxit_TRW = 3;
+    frecTRW = 1;
+    double *avg = (double *)malloc(sizeof(double) * 2 * nbits * NRNBETAS);
+    for (int bin = 0; bin < 2; bin++) {
+      for (int ib = 0; ib < nbits; ib++)
+        for (int k = 0; k < NRNBETAS; k++) {
+          const int e0 = E0[ib * NRNBETAS + k];
+          average_Energy[ib][k] = (double)e0 + (double)((k * 5 + ib + bin * 3) % 11) / 7.0 -
+                                  (bin ? 1e-3 * e0 : 0.0);
+          avg[(bin * nbits + ib) * NRNBETAS + k] = average_Energy[ib][k];
+        }
+      write_measures(bin, nbits);
+    }
+    dump("meas_avg", avg, sizeof(double) * 2 * nbits * NRNBETAS);
+    free(avg);
+    unsigned char *rows =
+        (unsigned char *)malloc((size_t)3 * nbits * NR * maxit_TRW * NBETAS);
+    for (int call = 0; call < 3; call++) {
+      const int ib_call = call == 0 ? 0 : 1;
+      for (int ib = 0; ib < nbits; ib++)
+        for (int r = 0; r < NR; r++)
+          for (int i = 0; i < maxit_TRW; i++)
+            for (int c = 0; c < NBETAS; c++) {
+              const int b = (which_beta_this_clon[ib][r][c] + i + 2 * call + r) % NBETAS;
+              history_betas[ib][r][i][c] = (unsigned char)b;
+              rows[((((size_t)call * nbits + ib) * NR + r) * maxit_TRW + i) * NBETAS + c] =
+                  (unsigned char)b;
+            }
+      write_history(ib_call, nbits);
+    }
+    dump("hist_rows", rows, (size_t)3 * nbits * NR * maxit_TRW * NBETAS);
+    free(rows);
+    char path[2 * MAXSTR];
+    const int last[2][2] = {{0, 0}, {nbits - 1, NR - 1}};
+    const char *tag[2] = {"0_0", "last"};
+    for (int k = 0; k < 2; k++) {
+      char nm[64];
+      snprintf(path, sizeof path, "%sevol.txt", dir_rep[last[k][0]][last[k][1]]);
+      snprintf(nm, sizeof nm, "evol_%s", tag[k]);
+      copy_file(path, nm);
+      snprintf(path, sizeof path, "%saceptancePT.txt", dir_rep[last[k][0]][last[k][1]]);
+      snprintf(nm, sizeof nm, "aceptancePT_%s", tag[k]);
+      copy_file(path, nm);
+      snprintf(path, sizeof path, "%shistory_b.dat", dir_rep[last[k][0]][last[k][1]]);
+      snprintf(nm, sizeof nm, "history_%s", tag[k]);
+      copy_file(path, nm);
+    }
+  }
+
   /* cluster stream (random_c) draws and Lüscher-derived seed lists */
   {
     uint64_t draws[64];

@@ -203,3 +203,45 @@ def test_tiled_sweeps_at_the_widest_rows():
     for shp in ((64, 1024, 2, 0), (64, 512, 2, 1)):
         say("L=64 shape", shp)
         same(want, sim(shp), "L=64 shape %s" % (shp,))
+
+
+def test_full_48_tube_mix():
+    """The batched workload at its stated size: 48 tubes, L from {8, 16,
+    24}, Lz from {L, 4L, 8L}, half PBC and half OBC, 4 replicas, 24
+    temperatures in [0.6, 1.6], per-tube seeds, sweep + PT + Houdayer per
+    step.  After 3 steps in one batch every tube equals the same tube run
+    alone; the largest PBC and OBC tubes also equal the oracle."""
+    import bench
+    tubes = bench.mixed_tubes(48)
+    nt = 24
+    specs = [(L, Lz, bc, nt, 1) for (L, Lz, bc) in tubes]
+    say("48 tubes:", sorted(set((L, Lz) for L, Lz, _ in tubes)))
+    plain = [make(s, 100 + k) for k, s in enumerate(specs)]
+    twins = [make(s, 100 + k) for k, s in enumerate(specs)]
+    batch = eatube.Batch([t.engine for t in twins])
+    batch.run([t.schedule(3, cluster=True, frec_trw=1) for t in twins])
+    for p in plain:
+        p.steps(3, cluster=True, frec_trw=1)
+    for k, (p, t) in enumerate(zip(plain, twins)):
+        same(state(p), state(t), "tube %d %s" % (k, specs[k]))
+    batch.close()
+    picks = []
+    for bc in (0, 1):
+        k = max((k for k in range(48) if specs[k][2] == bc),
+                key=lambda k: (specs[k][0] ** 2 * specs[k][1], -k))
+        picks.append(k)
+    orc.set_threads(8)
+    try:
+        for k in picks:
+            L, Lz, bc, nt_, ns = specs[k]
+            say("oracle tube", k, specs[k])
+            o = orc.Sim(L, Lz, 4, nt_, ns, bc=bc,
+                        betas=orc.betas_linear(0.6, 1.6, nt_),
+                        seeds=SEEDS(100 + k), job_id=100 + k)
+            o.steps(3, cluster=True)
+            t = twins[k]
+            assert np.array_equal(t.spins(), o.spins()), "tube %d spins" % k
+            assert np.array_equal(t.engine.download_perms().ravel(),
+                                  o.field("wctb")), "tube %d perms" % k
+    finally:
+        orc.set_threads(1)

@@ -240,8 +240,10 @@ def test_full_size_houdayer_involution_and_conservation(bc):
 
 def test_full_size_houdayer_31_samples():
     """L=48 Lz=384 with all 31 samples: 992 cluster instances, each
-    labelled in 24 slabs joined at their seams.  Labels of the last
-    instances equal the oracle's, and the move is an involution."""
+    labelled in 24 slabs joined at their seams.  The labels of every
+    instance equal the oracle's breadth-first minimum-index components, and
+    the move is an involution."""
+    from concurrent.futures import ThreadPoolExecutor
     e = full_size_sim(bc=1, ns=31)
     eng = e.engine
     e.steps(1, cluster=False)
@@ -251,20 +253,50 @@ def test_full_size_houdayer_31_samples():
     eng.keep_labels()
     eng.houdayer(pairs, 4242, 0)
     lab = eng.download_labels()
-    say("labels downloaded")
+    say("labels downloaded:", lab.shape)
     L, Lz, ctm = e.L, e.Lz, eng.ctm
-    for (grp, s, b) in [(1, 30, ctm - 1), (0, 20, 3), (1, 15, 0)]:
+
+    def check(inst):
+        grp, s, b = inst
         r1, r2 = pairs[2 * grp], pairs[2 * grp + 1]
         c1, c2 = w[s, r1, b], w[s, r2, b]
         u1 = eatube.unpack_lattice(L, Lz, e.nr, e.nt, before[s, 0].ravel(),
                                    before[s, 1].ravel(), r1, c1)
         u2 = eatube.unpack_lattice(L, Lz, e.nr, e.nt, before[s, 0].ravel(),
                                    before[s, 1].ravel(), r2, c2)
-        want = orc.labels(L, Lz, 1, u1 ^ u2)
-        assert np.array_equal(lab[grp, s, b], want), (grp, s, b)
+        return np.array_equal(lab[grp, s, b], orc.labels(L, Lz, 1, u1 ^ u2))
+
+    insts = [(g, s, b) for g in (0, 1) for s in range(e.ns)
+             for b in range(ctm)]
+    assert len(insts) == 992
+    with ThreadPoolExecutor(16) as ex:
+        ok = list(ex.map(check, insts))
+    bad = [i for i, k in zip(insts, ok) if not k]
+    assert not bad, bad[:5]
+    say("all %d instances' labels equal the oracle's" % len(insts))
     del lab
     eng.houdayer(pairs, 4242, 0)
     assert np.array_equal(e.spins(), before)
+
+
+def test_full_size_step_matches_oracle():
+    """One step of the large workload (L=48 Lz=384 PBC, 4 replicas, 32
+    temperatures in [0.5, 1.6]; sweep + PT + Houdayer + measure) for two
+    samples against the oracle: spins, PT permutations, generators and
+    statistics, energy sums, energies and every cluster label."""
+    o, e = pair(48, 384, 4, 32, 2, bc=0, tmin=0.5, tmax=1.6,
+                seeds=(4848, 384, 44, 3232))
+    o.keep_labels()
+    e.engine.keep_labels()
+    orc.set_threads(8)
+    try:
+        e.steps(1, cluster=True)
+        o.steps(1, cluster=True)
+    finally:
+        orc.set_threads(1)
+    assert_same_state(o, e, "large step")
+    assert np.array_equal(e.engine.download_labels().ravel(),
+                          o.field("last_labels")), "labels"
 
 
 def test_full_size_pt_matches_oracle_pt():
@@ -424,3 +456,27 @@ def test_shape_change_after_an_empty_run_sums_pending_energies():
     assert np.array_equal(acc.ravel(), o.field("acc"))
     assert np.array_equal(raw.ravel(), o.field("raw"))
     assert np.array_equal(e.engine.energy().ravel(), o.field("ener"))
+
+
+@pytest.mark.parametrize("ptfr,houfr,mesfr", [(2, 1, 1), (1, 2, 1), (1, 1, 2),
+                                              (2, 2, 2)])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_schedules_match_oracle(ptfr, houfr, mesfr, bc):
+    """ea_run's loop nesting (main.cu:149-228): mesfr x { houfr x { ptfr
+    sweeps, PT }, Houdayer }, energy sums and TRW history, against the
+    oracle's it loop: spins, perms, PT generators and statistics, energy
+    sums, energies and history rows.  A second call continues the bin
+    (it0 > 0) and grows the history; the rows of the first call stay."""
+    o, e = pair(16, 16, 4, 8, 3, bc=bc, seeds=(51, 52, 53, 54 + bc))
+    kw = dict(ptfr=ptfr, houfr=houfr, mesfr=mesfr, cluster=True)
+    e.steps(5, it0=1, frec_trw=2, **kw)
+    h1 = o.run(5, it0=1, frec_trw=2, rows=3, **kw)
+    assert_same_state(o, e, "first call")  # reads and clears the sums
+    _, eh = e.engine.read_measures(rows=3)
+    assert np.array_equal(eh, h1), "history rows of the first call"
+    e.steps(4, it0=6, frec_trw=2, **kw)
+    h2 = o.run(4, it0=6, frec_trw=2, rows=5, **kw)
+    assert_same_state(o, e, "second call")
+    _, eh = e.engine.read_measures(rows=5)
+    assert np.array_equal(eh[:3], h1), "rows of the first call kept"
+    assert np.array_equal(eh[3:], h2[3:]), "rows of the second call"

@@ -258,3 +258,40 @@ def test_overlap_is_the_sum_over_unpacked_lattices(bc):
                     assert got[s, k, b] == int(np.sum(a * c)), (s, r1, r2, b)
                 k += 1
     assert np.all(np.abs(got) <= V) and np.any(got != V)
+
+
+@pytest.mark.parametrize("ptfr,houfr,mesfr", [(1, 1, 1), (2, 1, 3), (1, 3, 2),
+                                              (2, 2, 2)])
+def test_it_loop_counts(ptfr, houfr, mesfr):
+    """The counts main.cu:149-228 fixes per `it`, for orc_sim_run: 2*ptfr
+    half-sweeps per PT, houfr PTs per Houdayer, mesfr Houdayer calls, one
+    energy sum per `it`, and a TRW row every frecTRW iterations holding
+    which_beta_this_clon at that iteration."""
+    n_it, nr, nt, ns, V = 3, 4, 5, 2, 8 * 8 * 8
+    betas = orc.betas_linear(0.6, 1.6, nt)
+    o = orc.Sim(8, 8, nr, nt, ns, bc=1, betas=betas, seeds=(5, 6, 7, 8))
+    hist = o.run(n_it, ptfr=ptfr, houfr=houfr, mesfr=mesfr, cluster=True,
+                 it0=2, frec_trw=2, rows=3)
+    calls = n_it * mesfr
+    assert o.field("iter")[0] == 2 * n_it * mesfr * houfr * ptfr
+    assert np.all(o.field("att") == n_it * mesfr * houfr)
+    assert o.field("rng_offset")[0] == calls * 2 * V * nt * ns
+    # each call draws nr/2 - 1 numbers per replica group (main.cu:409-421)
+    cl = orc.hq_init(8)
+    for _ in range(calls * 2 * (nr // 2 - 1)):
+        orc.lib().orc_hq_next(C.byref(cl))
+    assert orc.hq_to_bytes(cl) == bytes(o.field("cl"))
+    # it0 = 2, frec 2: iterations 2, 3, 4 -> a row at it = 3 (row 1) only;
+    # it = 3 is the middle iteration, so the row is not the final state
+    assert not hist[0].any() and not hist[2].any()
+    assert all(sorted(hist[1, s, r]) == list(range(nt))
+               for s in range(ns) for r in range(nr))
+    # the sums hold one energy per `it` and beta: for a fresh run with one
+    # iteration they are the energies at the end of that iteration
+    o2 = orc.Sim(8, 8, nr, nt, ns, bc=1, betas=betas, seeds=(5, 6, 7, 8))
+    h2 = o2.run(1, ptfr=ptfr, houfr=houfr, mesfr=mesfr, cluster=True, rows=1)
+    e = o2.field("ener").reshape(ns, nr, nt)
+    w = o2.field("wctb").reshape(ns, nr, nt)
+    want = np.take_along_axis(e, w.astype(np.int64), axis=2)
+    assert np.array_equal(o2.field("esum").reshape(ns, nr, nt), want)
+    assert np.array_equal(h2[0], o2.field("wbtc").reshape(ns, nr, nt))

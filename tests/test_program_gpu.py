@@ -134,3 +134,104 @@ def test_program_refuses_a_second_new_run(tmp_path):
                        timeout=300)
     assert r.returncode == 1 and "There are confs" in r.stderr
     assert (tmp_path / "out" / "I000" / "error").exists()
+
+
+def frec_trw(itmax):
+    """frecTRW and maxit_TRW as read_input picks them (io.c:431-434)."""
+    for f in range(1, itmax + 1):
+        if itmax % f == 0 and itmax // f <= 512:
+            return f, itmax // f
+
+
+class OracleFiles:
+    """The program's measurement files as the oracle's run of the same
+    seeds gives them, formatted by the reference's writers (orc_files)."""
+
+    def __init__(self, itmax, mesfr, ptfr, houfr):
+        import orc
+        self.o = orc.Sim(L, LZ, NR, NT, NBITS, bc=1, seeds=SEEDS, job_id=0,
+                         betas=np.array(BETAS))
+        self.k = (itmax, mesfr, ptfr, houfr)
+        self.frec, self.maxit = frec_trw(itmax)
+        self.evol, self.acc, self.hist = {}, {}, {}
+
+    def bin(self, ib):
+        import orc_files
+        itmax, mesfr, ptfr, houfr = self.k
+        o = self.o
+        rows = o.run(itmax, ptfr=ptfr, houfr=houfr, mesfr=mesfr, cluster=True,
+                     it0=0, frec_trw=self.frec, rows=self.maxit)
+        avg = o.field("esum").reshape(NBITS, NR, NT) / itmax
+        o.field("esum")[:] = 0
+        acc, att, raw = (o.field(k).reshape(NBITS, NR, NT - 1)
+                         for k in ("acc", "att", "raw"))
+        for bit in range(NBITS):
+            for r in range(NR):
+                key = (bit, r)
+                self.evol[key] = self.evol.get(key, "") + \
+                    orc_files.evol_line(ib, avg[bit, r])
+                self.acc[key] = self.acc.get(key, "") + \
+                    orc_files.acceptance_block(ib, BETAS, acc[bit, r],
+                                               raw[bit, r], att[bit, r])
+                self.hist[key] = orc_files.history_append(
+                    self.hist.get(key, b""), ib, rows[:, bit, r])
+
+    def check(self, tmp, accept=True):
+        for bit in range(NBITS):
+            for r in range(NR):
+                d = rep_dir(tmp, bit, r)
+                assert (d / "evol.txt").read_text() == self.evol[bit, r], \
+                    (bit, r, "evol.txt")
+                if accept:
+                    assert (d / "aceptancePT.txt").read_text() == \
+                        self.acc[bit, r], (bit, r, "aceptancePT.txt")
+                assert (d / "history_b.dat").read_bytes() == \
+                    self.hist[bit, r], (bit, r, "history_b.dat")
+
+
+@pytest.mark.parametrize("itmax,mesfr,ptfr,houfr", [
+    (4, 2, 2, 1), (3, 1, 1, 2), (2, 2, 1, 2), (1024, 1, 1, 1)])
+def test_program_files_match_oracle(tmp_path, itmax, mesfr, ptfr, houfr):
+    """evol.txt, aceptancePT.txt and history_b.dat of a two-bin run, byte
+    for byte, against the oracle's energies, PT statistics and TRW rows
+    written as the reference's write_measures / write_history write them
+    (itmax = 1024 stores every second iteration: frecTRW = 2)."""
+    setup(tmp_path, nbin=2, flag=1, itmax=itmax, mesfr=mesfr, ptfr=ptfr,
+          houfr=houfr)
+    run(tmp_path)
+    want = OracleFiles(itmax, mesfr, ptfr, houfr)
+    for ib in range(2):
+        want.bin(ib)
+    want.check(tmp_path)
+
+
+def test_restart_counter_omits_houfr(tmp_path):
+    """With houfr = 2 the reference restarts its half-sweep counter at
+    2*itcut*itmax*mesfr*ptfr (main.cu:139), not at the count the first bin
+    reached, so a resumed run continues on other random words than an
+    uninterrupted one.  The program keeps that: its resumed run equals the
+    oracle run whose counter is reset that way after the first bin (PT
+    statistics, which the conf does not hold, restart from zero)."""
+    itmax, mesfr, ptfr, houfr = 3, 1, 2, 2
+    a, b = tmp_path / "resumed", tmp_path / "straight"
+    a.mkdir()
+    b.mkdir()
+    setup(a, nbin=1, flag=1, itmax=itmax, mesfr=mesfr, ptfr=ptfr, houfr=houfr)
+    run(a)
+    setup(a, nbin=2, flag=2, itmax=itmax, mesfr=mesfr, ptfr=ptfr, houfr=houfr)
+    run(a)
+    want = OracleFiles(itmax, mesfr, ptfr, houfr)
+    want.bin(0)
+    o = want.o
+    assert o.field("iter")[0] == 2 * itmax * mesfr * houfr * ptfr
+    o.field("iter")[0] = 2 * 1 * itmax * mesfr * ptfr
+    for k in ("acc", "att", "raw"):
+        o.field(k)[:] = 0
+    want.bin(1)
+    want.check(a)
+    # the uninterrupted run differs: the quirk matters at houfr > 1
+    setup(b, nbin=2, flag=1, itmax=itmax, mesfr=mesfr, ptfr=ptfr, houfr=houfr)
+    run(b)
+    assert any((rep_dir(a, bit, r) / "evol.txt").read_bytes() !=
+               (rep_dir(b, bit, r) / "evol.txt").read_bytes()
+               for bit in range(NBITS) for r in range(NR))

@@ -45,3 +45,26 @@ def test_mix_equals_alone():
         ref = alone(r)
         for k in ("spins", "perms", "esum", "energies"):
             assert np.array_equal(getattr(x, k), ref[k]), (r.rid, k)
+
+
+def test_poisson_stream_at_the_serving_shapes():
+    """Requests drawn as the serving workload draws them (L in {8, 16, 24,
+    32}, Lz a multiple of 8 in [L, 16L], PBC or OBC, NT in {16, 24, 32},
+    sweeps log-uniform in [1e3, 1e5], Houdayer moves), with sweeps capped at
+    1500 to fit the test's time, plus the largest shape (L = 32, Lz = 512,
+    NT = 32): each result of the mix equals the request run alone."""
+    reqs = serve.poisson_requests(12, rate=40.0, seed=11)
+    for r in reqs:
+        r.sweeps = min(r.sweeps, 1500)
+    reqs.append(serve.Request(rid=12, seed=99, L=32, Lz=512, bc=1, nt=32,
+                              sweeps=1000, arrival=reqs[-1].arrival))
+    srv = serve.Server(budget=8 << 30, chunk=64, keep=True, workers=4)
+    res = srv.run(reqs)
+    print("stats", srv.stats, flush=True)
+    assert srv.stats["max_running"] >= 2
+    for r, x in zip(reqs, res):
+        print("request", r.rid, (r.L, r.Lz, r.bc, r.nt, r.sweeps),
+              "latency %.3f s" % x.latency, flush=True)
+        ref = alone(r)
+        for k in ("spins", "perms", "esum", "energies"):
+            assert np.array_equal(getattr(x, k), ref[k]), (r.rid, k)
