@@ -80,7 +80,12 @@ struct ea_engine {
   Shape shape;          // launch shape of the engine's own calls
   uint32_t job = 0;
   std::vector<double> betas;
-  size_t ctr_cap = 0, pairs_cap = 0;
+  // Call block: the descriptor, the step counter and the inputs of the calls
+  // in flight, on the device (`dt` points at its start) and in a pinned host
+  // mirror from which one asynchronous copy sends them (stage)
+  uint8_t *call_dev = nullptr, *call_host = nullptr;
+  size_t call_cap = 0;
+  hipEvent_t staged = nullptr;  // recorded after the last staging copy
   ea_counter *noalloc_ctr = nullptr;  // this device's `scratch` address (not owned)
   int hist_rows = 0;
   int keep_on = 0;
@@ -119,12 +124,14 @@ struct ea_engine {
     }
     for (auto ev : spare) (void)hipEventDestroy(ev);
     free_plan(&plan);
-    const void *ptrs[] = {t.spin, t.J,    t.wctb, t.lut,  t.ener, t.part,     t.pt,
-                          t.acc,  t.att,  t.raw,  t.nd,   t.tab,  t.toff,     t.tlen,
-                          t.esum, t.hist, t.hpar, t.hl16, t.hrow, t.hflag,    t.step,
-                          t.ctr,  t.pairs, t.ovl,  keep_buf, dt};
+    const void *ptrs[] = {t.spin, t.J,    t.wctb, t.lut,  t.ener, t.part,   t.pt,
+                          t.acc,  t.att,  t.raw,  t.nd,   t.tab,  t.toff,   t.tlen,
+                          t.esum, t.hist, t.hpar, t.hl16, t.hrow, t.hflag,  t.ovl,
+                          keep_buf, call_dev};
     for (const void *p : ptrs)
       if (p) (void)hipFree(const_cast<void *>(p));
+    if (call_host) (void)hipHostFree(call_host);
+    if (staged) (void)hipEventDestroy(staged);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -221,71 +228,98 @@ static int check_config(const ea_config *c) {
   return EA_OK;
 }
 
-// Waits for the engine's stream and copies the host descriptor to the device.
-static int upload_tube(ea_engine *e) {
+// Layout of an engine's call block: the descriptor, the step counter, then
+// the counter blocks and the replica pairs of the calls in flight.
+constexpr size_t kOffStep = (sizeof(Tube) + 15) / 16 * 16;
+constexpr size_t kOffCtr = kOffStep + 16;
+constexpr size_t kCallInitial = 64 * 1024;  // bytes allocated at creation
+
+inline size_t align16(size_t x) { return (x + 15) / 16 * 16; }
+
+// Grows both copies of the call block to at least `bytes`; every call
+// restages what it needs, so nothing is carried over.
+static int grow_call_block(ea_engine *e, size_t bytes) {
+  if (bytes <= e->call_cap) return EA_OK;
+  bytes = std::max(bytes, 2 * e->call_cap);
   HIP_TRY(hipStreamSynchronize(e->stream));
-  HIP_TRY(hipMemcpy(e->dt, &e->t, sizeof(Tube), hipMemcpyHostToDevice));
+  uint8_t *d = nullptr, *h = nullptr;
+  RC_TRY(dalloc(&d, bytes));
+  if (hipHostMalloc(reinterpret_cast<void **>(&h), bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipFree(d);
+    return fail(EA_E_NOMEM, "hipHostMalloc of %zu bytes failed", bytes);
+  }
+  if (e->call_dev) HIP_TRY(hipFree(e->call_dev));
+  if (e->call_host) HIP_TRY(hipHostFree(e->call_host));
+  e->call_dev = d;
+  e->call_host = h;
+  e->call_cap = bytes;
+  e->dt = reinterpret_cast<Tube *>(d);
+  e->t.step = reinterpret_cast<int *>(d + kOffStep);
   return EA_OK;
 }
 
+// Sends the host descriptor to the device -- with `reset` also the step
+// counter (0) and the inputs of the calls in flight -- in one asynchronous
+// copy on the engine's stream: it lands after the work queued before it, and
+// the calls' launches follow without the host waiting.  The host mirror is
+// reused once the previous copy has been made.
+static int stage(ea_engine *e, bool reset, const ea_counter *ctr = nullptr, size_t nctr = 0,
+                 const int32_t *pairs = nullptr, size_t npairs = 0) {
+  const size_t off_pairs = align16(kOffCtr + sizeof(ea_counter) * nctr);
+  const size_t end = reset ? off_pairs + sizeof(int32_t) * npairs : sizeof(Tube);
+  RC_TRY(grow_call_block(e, end));
+  HIP_TRY(hipEventSynchronize(e->staged));
+  uint8_t *h = e->call_host;
+  if (reset) {
+    if (nctr) e->t.ctr = reinterpret_cast<const ea_counter *>(e->call_dev + kOffCtr);
+    if (npairs) e->t.pairs = reinterpret_cast<const int32_t *>(e->call_dev + off_pairs);
+    const int zero = 0;
+    std::memcpy(h + kOffStep, &zero, sizeof zero);
+    if (nctr) std::memcpy(h + kOffCtr, ctr, sizeof(ea_counter) * nctr);
+    if (npairs) std::memcpy(h + off_pairs, pairs, sizeof(int32_t) * npairs);
+  }
+  std::memcpy(h, &e->t, sizeof(Tube));
+  HIP_TRY(hipMemcpyAsync(e->call_dev, h, end, hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipEventRecord(e->staged, e->stream));
+  return EA_OK;
+}
+
+// The descriptor alone (the step counter keeps its value).
+static int upload_tube(ea_engine *e) { return stage(e, false); }
+
 // Sets up the calls in flight: nsteps steps of `halves` half-sweeps and
-// `calls` Houdayer calls, step counter at 0.
-static int begin_calls(ea_engine *e, int nsteps, int halves, int calls, int it0, int frec) {
+// `calls` Houdayer calls, their counter blocks and replica pairs, step
+// counter at 0.
+static int begin_calls(ea_engine *e, int nsteps, int halves, int calls, int it0, int frec,
+                       const ea_counter *ctr = nullptr, size_t nctr = 0,
+                       const int32_t *pairs = nullptr, size_t npairs = 0) {
   e->t.nsteps = nsteps;
   e->t.halves = halves;
   e->t.calls = calls;
   e->t.it0 = it0;
   e->t.frec = std::max(frec, 1);
-  HIP_TRY(hipStreamSynchronize(e->stream));
-  const int zero = 0;
-  HIP_TRY(hipMemcpy(e->t.step, &zero, sizeof zero, hipMemcpyHostToDevice));
-  return upload_tube(e);
+  return stage(e, true, ctr, nctr, pairs, npairs);
 }
 
-// Host inputs of the calls in flight: counter blocks and replica pairs.
-static int stage_inputs(ea_engine *e, const ea_counter *ctr, size_t nctr, const int32_t *pairs,
-                        size_t npairs) {
-  HIP_TRY(hipStreamSynchronize(e->stream));
-  if (nctr > e->ctr_cap) {
-    if (e->t.ctr) HIP_TRY(hipFree(const_cast<ea_counter *>(e->t.ctr)));
-    e->t.ctr = nullptr;
-    e->ctr_cap = 0;
-    ea_counter *p = nullptr;
-    RC_TRY(dalloc(&p, nctr));
-    e->t.ctr = p;
-    e->ctr_cap = nctr;
-  }
-  if (npairs > e->pairs_cap) {
-    if (e->t.pairs) HIP_TRY(hipFree(const_cast<int32_t *>(e->t.pairs)));
-    e->t.pairs = nullptr;
-    e->pairs_cap = 0;
-    int32_t *p = nullptr;
-    RC_TRY(dalloc(&p, npairs));
-    e->t.pairs = p;
-    e->pairs_cap = npairs;
-  }
-  if (nctr)
-    HIP_TRY(hipMemcpy(const_cast<ea_counter *>(e->t.ctr), ctr, sizeof(ea_counter) * nctr,
-                      hipMemcpyHostToDevice));
-  if (npairs)
-    HIP_TRY(hipMemcpy(const_cast<int32_t *>(e->t.pairs), pairs, sizeof(int32_t) * npairs,
-                      hipMemcpyHostToDevice));
-  return EA_OK;
-}
-
-// Grows the TRW history to `rows` rows; the rows written so far are kept
-// (a caller may advance it0 across ea_run calls of one bin), new rows are 0.
-// On failure the old buffer stays in place.
+// Grows the TRW history to hold `rows` rows; the rows written so far are
+// kept (a caller may advance it0 across ea_run calls of one bin), new rows
+// are 0.  It grows to at least 512 rows (the reference's MAXIT_TRW rows per
+// bin) and at least doubles, so that a run's calls rarely allocate.  On
+// failure the old buffer stays in place.
 static int ensure_hist(ea_engine *e, int rows) {
   const Lattice &g = e->g();
   if (rows <= e->hist_rows) return EA_OK;
+  rows = std::max({rows, 2 * e->hist_rows, 512});
   const size_t per = (size_t)g.ns * g.nr * g.nt;
-  HIP_TRY(hipStreamSynchronize(e->stream));
   uint8_t *h = nullptr;
   RC_TRY(dalloc(&h, (size_t)rows * per));
-  hipError_t err = hipMemset(h, 0, (size_t)rows * per);
+  // on the engine's stream: the clear and the copy land after the work that
+  // wrote the old rows and before the calls that write new ones
+  hipError_t err = hipMemsetAsync(h, 0, (size_t)rows * per, e->stream);
   if (err == hipSuccess && e->t.hist)
-    err = hipMemcpy(h, e->t.hist, (size_t)e->hist_rows * per, hipMemcpyDeviceToDevice);
+    err = hipMemcpyAsync(h, e->t.hist, (size_t)e->hist_rows * per, hipMemcpyDeviceToDevice,
+                         e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
   if (err != hipSuccess) {
     (void)hipFree(h);
     return fail(EA_E_HIP, "growing the history: %s", hipGetErrorString(err));
@@ -350,6 +384,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   e->betas.assign(cfg->betas, cfg->betas + cfg->nt);
 
   HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&e->staged, hipEventDisableTiming));
   const size_t chains = (size_t)g.ns * g.nr;
   uint64_t *J = nullptr;
   uint8_t *lut = nullptr;
@@ -361,27 +396,26 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
       (rc = dalloc(&t.acc, chains * (g.nt - 1))) ||
       (rc = dalloc(&t.att, chains * (g.nt - 1))) ||
       (rc = dalloc(&t.raw, chains * (g.nt - 1))) || (rc = dalloc(&t.esum, chains * g.nt)) ||
-      (rc = dalloc(&t.step, 1)) || (rc = dalloc(&e->dt, 1))) {
+      (rc = grow_call_block(e.get(), kCallInitial))) {
     t.J = J;
     t.lut = lut;
     return rc;
   }
   t.J = J;
   t.lut = lut;
-  HIP_TRY(hipMemset(t.spin, 0, sizeof(uint64_t) * g.ns * 2 * g.half));
-  HIP_TRY(hipMemset(J, 0, sizeof(uint64_t) * g.ns * 12 * g.n));
+  HIP_TRY(hipMemsetAsync(t.spin, 0, sizeof(uint64_t) * g.ns * 2 * g.half, e->stream));
+  HIP_TRY(hipMemsetAsync(J, 0, sizeof(uint64_t) * g.ns * 12 * g.n, e->stream));
   HIP_TRY(hipMemcpy(lut, cfg->lut, (size_t)g.nt * EA_LUT_BYTES, hipMemcpyHostToDevice));
   std::vector<uint8_t> ident(chains * g.nt);
   for (size_t c = 0; c < chains; ++c)
     for (int b = 0; b < g.nt; ++b) ident[c * g.nt + b] = (uint8_t)b;
   HIP_TRY(hipMemcpy(t.wctb, ident.data(), ident.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemset(t.ener, 0, sizeof(int32_t) * chains * g.nt));
-  HIP_TRY(hipMemset(t.pt, 0, sizeof(ea_hq_state) * chains));
-  HIP_TRY(hipMemset(t.acc, 0, sizeof(double) * chains * (g.nt - 1)));
-  HIP_TRY(hipMemset(t.att, 0, sizeof(double) * chains * (g.nt - 1)));
-  HIP_TRY(hipMemset(t.raw, 0, sizeof(double) * chains * (g.nt - 1)));
-  HIP_TRY(hipMemset(t.esum, 0, sizeof(double) * chains * g.nt));
-  HIP_TRY(hipMemset(t.step, 0, sizeof(int)));
+  HIP_TRY(hipMemsetAsync(t.ener, 0, sizeof(int32_t) * chains * g.nt, e->stream));
+  HIP_TRY(hipMemsetAsync(t.pt, 0, sizeof(ea_hq_state) * chains, e->stream));
+  HIP_TRY(hipMemsetAsync(t.acc, 0, sizeof(double) * chains * (g.nt - 1), e->stream));
+  HIP_TRY(hipMemsetAsync(t.att, 0, sizeof(double) * chains * (g.nt - 1), e->stream));
+  HIP_TRY(hipMemsetAsync(t.raw, 0, sizeof(double) * chains * (g.nt - 1), e->stream));
+  HIP_TRY(hipMemsetAsync(t.esum, 0, sizeof(double) * chains * g.nt, e->stream));
 
   // exact exp(pt) tables for the device parallel tempering
   const ExpTables x = make_exp_tables(cfg->betas, g.nt, g.V);
@@ -425,7 +459,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
     }
   }
   HIP_TRY(upload_plan(make_plan({t}, e->shape), &e->plan));
-  RC_TRY(upload_tube(e.get()));
+  RC_TRY(begin_calls(e.get(), 0, 0, 0, 0, 1));
   *out = e.release();
   return EA_OK;
 }
@@ -447,7 +481,7 @@ static uint64_t engine_bytes(const ea_config *cfg, int lds_max, int lds_cu, int 
   uint64_t b = 8ull * g.ns * 2 * g.half + 8ull * g.ns * 12 * g.n + chains * g.nt +
                (uint64_t)g.nt * EA_LUT_BYTES + 4ull * chains * g.nt + sizeof(ea_hq_state) * chains +
                4ull * energy_partials(g) + 24ull * chains * (g.nt - 1) + 8ull * chains * g.nt +
-               sizeof(int) + sizeof(Tube);
+               kCallInitial;
   const ExpTables x = make_exp_tables(cfg->betas, g.nt, g.V);
   b += 8ull * x.nd.size() + 8ull * x.off.size() + 8ull * std::max<size_t>(x.val.size(), 1);
   if (g.ctm > 0 && hou_use_slabs(g, lds_max, lds_cu, cus))
@@ -663,14 +697,9 @@ static int ensure_energy(ea_engine *e) {
   return EA_OK;
 }
 
-// Enqueues the 2*nsweeps half-sweeps of one call whose counter blocks are
-// already on the device at `ctr` (timed by ea_set_timing when `timed`).
-static int enqueue_sweeps(ea_engine *e, int nsweeps, const ea_counter *ctr, bool timed) {
-  const ea_counter *own = e->t.ctr;
-  e->t.ctr = ctr;  // the device copy of the descriptor carries it
-  const int rc = begin_calls(e, 1, 2 * nsweeps, 0, 0, 1);
-  e->t.ctr = own;
-  RC_TRY(rc);
+// Enqueues the 2*nsweeps half-sweeps of the call in flight (timed by
+// ea_set_timing when `timed`).
+static int enqueue_sweeps(ea_engine *e, int nsweeps, bool timed) {
   for (int half = 0; half < 2 * nsweeps; ++half) {
     Phase ph{half, half == 2 * nsweeps - 1, 0, 0, 0};
     Timed t(timed ? e : nullptr, EA_K_SWEEP);
@@ -685,8 +714,8 @@ extern "C" int ea_sweep(ea_engine *e, int nsweeps, const ea_counter *blocks) {
   if (!e || (nsweeps > 0 && !blocks)) return fail(EA_E_ARG, "NULL argument");
   if (nsweeps < 0) return fail(EA_E_ARG, "nsweeps < 0");
   if (nsweeps == 0) return EA_OK;
-  RC_TRY(stage_inputs(e, blocks, (size_t)nsweeps * 2 * e->g().nr, nullptr, 0));
-  return enqueue_sweeps(e, nsweeps, e->t.ctr, true);
+  RC_TRY(begin_calls(e, 1, 2 * nsweeps, 0, 0, 1, blocks, (size_t)nsweeps * 2 * e->g().nr));
+  return enqueue_sweeps(e, nsweeps, true);
 }
 
 extern "C" int ea_sweep_noalloc(ea_engine *e, int nsweeps, const ea_counter *blocks) {
@@ -703,7 +732,14 @@ extern "C" int ea_sweep_noalloc(ea_engine *e, int nsweeps, const ea_counter *blo
   // the engine's previous kernels may still read the buffer
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipMemcpy(e->noalloc_ctr, blocks, sizeof(ea_counter) * nctr, hipMemcpyHostToDevice));
-  return enqueue_sweeps(e, nsweeps, e->noalloc_ctr, false);
+  // the descriptor points the kernels at the shared buffer; staging it
+  // without inputs stays within the call block allocated at creation
+  const ea_counter *own = e->t.ctr;
+  e->t.ctr = e->noalloc_ctr;
+  const int rc = begin_calls(e, 1, 2 * nsweeps, 0, 0, 1);
+  e->t.ctr = own;
+  RC_TRY(rc);
+  return enqueue_sweeps(e, nsweeps, false);
 }
 
 extern "C" int ea_energy(ea_engine *e, int32_t *out) {
@@ -829,11 +865,10 @@ extern "C" int ea_houdayer(ea_engine *e, const int32_t pairs[4], uint64_t coin_s
     return fail(EA_E_STATE, "the Houdayer move needs nr >= 4 and clone_threshold > 0");
   RC_TRY(check_pairs(e, pairs));
   RC_TRY(set_cluster(e, 1));
-  RC_TRY(stage_inputs(e, nullptr, 0, pairs, 4));
   e->t.coin_seed = coin_seed;
   e->t.coin0 = *rng_offset;
   e->t.coin_per_call = coin_per_call(e->g());
-  RC_TRY(begin_calls(e, 1, 0, 1, 0, 1));
+  RC_TRY(begin_calls(e, 1, 0, 1, 0, 1, nullptr, 0, pairs, 4));
   RC_TRY(ensure_energy(e));  // the move updates the two changed chains' energies
   {
     Timed t(e, EA_K_HOUDAYER);
@@ -881,14 +916,13 @@ static int load_schedule(ea_engine *e, const ea_schedule *sch) {
   const Lattice &g = e->g();
   const int halves = 2 * sch->ptfr * sch->houfr * sch->mesfr;
   RC_TRY(ensure_hist(e, (sch->it0 + sch->n_it) / sch->frec_trw));
-  RC_TRY(stage_inputs(e, sch->blocks, (size_t)sch->n_it * halves * g.nr,
-                      sch->cluster ? sch->pairs : nullptr,
-                      sch->cluster ? (size_t)sch->n_it * sch->mesfr * 4 : 0));
   RC_TRY(set_cluster(e, sch->cluster ? 1 : 0));
   e->t.coin_seed = sch->coin_seed;
   e->t.coin0 = sch->rng_offset;
   e->t.coin_per_call = coin_per_call(g);
-  return begin_calls(e, sch->n_it, halves, sch->mesfr, sch->it0, sch->frec_trw);
+  return begin_calls(e, sch->n_it, halves, sch->mesfr, sch->it0, sch->frec_trw, sch->blocks,
+                     (size_t)sch->n_it * halves * g.nr, sch->cluster ? sch->pairs : nullptr,
+                     sch->cluster ? (size_t)sch->n_it * sch->mesfr * 4 : 0);
 }
 
 static void finish_schedule(ea_engine *e, const ea_schedule *sch, uint64_t *rng_offset_out) {
@@ -1035,6 +1069,9 @@ extern "C" int ea_batch_run(ea_batch *b, const ea_schedule *sch, uint64_t *rng_o
   }
   HIP_TRY(hipStreamSynchronize(b->stream));
   HIP_TRY(hipMemcpy(b->dt, tubes.data(), sizeof(Tube) * n, hipMemcpyHostToDevice));
+  // the engines' step counters and inputs reach their call blocks on the
+  // engines' own streams
+  for (int i = 0; i < n; ++i) HIP_TRY(hipStreamWaitEvent(b->stream, b->eng[i]->staged, 0));
   if (cluster != b->plan_cluster || !b->plan.ready() || !(b->plan.shape == b->launch)) {
     RC_TRY(drop_graph(b));
     HIP_TRY(upload_plan(make_plan(tubes, b->launch), &b->plan));
