@@ -338,6 +338,15 @@ static int pt_exact_from_env() {
   return v && v[0] == '1' && v[1] == '\0';
 }
 
+// EA_HOU_SITE_FOREST=1: the whole-tube cluster labelling always uses its
+// forest over sites, the path otherwise taken only by an instance with more
+// runs than the run forest holds.  Same labels either way; it lets that
+// path be tested.
+static bool site_forest_from_env() {
+  const char *v = std::getenv("EA_HOU_SITE_FOREST");
+  return v && v[0] == '1' && v[1] == '\0';
+}
+
 extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   if (!out) return fail(EA_E_ARG, "out is NULL");
   *out = nullptr;
@@ -380,6 +389,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   t.frec = 1;
   t.cluster = g.ctm > 0 && g.nr >= 4;
   t.pt_exact = pt_exact_from_env();
+  t.hou_cap = site_forest_from_env() ? 0 : hou_run_cap(g);
   e->job = cfg->job_id;
   e->betas.assign(cfg->betas, cfg->betas + cfg->nt);
 
@@ -816,8 +826,8 @@ static int check_shape(const ea_shape *sh) {
   if (!sh) return fail(EA_E_ARG, "NULL argument");
   if (sh->sweep_positions != 32 && sh->sweep_positions != 64)
     return fail(EA_E_ARG, "sweep_positions must be 32 or 64 (got %d)", sh->sweep_positions);
-  if (sh->cluster_threads != 256 && sh->cluster_threads != 512 && sh->cluster_threads != 1024)
-    return fail(EA_E_ARG, "cluster_threads must be 256, 512 or 1024 (got %d)",
+  if (sh->cluster_threads < 256 || sh->cluster_threads > 1024 || sh->cluster_threads % 64)
+    return fail(EA_E_ARG, "cluster_threads must be a multiple of 64 in 256..1024 (got %d)",
                 sh->cluster_threads);
   if (sh->sweep_kind < 0 || sh->sweep_kind > 2)
     return fail(EA_E_ARG, "sweep_kind must be 0, 1 or 2 (got %d)", sh->sweep_kind);

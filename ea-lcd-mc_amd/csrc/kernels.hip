@@ -446,6 +446,14 @@ __device__ __forceinline__ void lds_barrier() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// Within one wave: wait until this wave's LDS accesses issued so far are
+// done (another lane of the wave reads what they wrote next).
+__device__ __forceinline__ void lds_wave_sync() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Sum over the first `width` lanes of each width-lane group.
 template <int W>
 __device__ __forceinline__ int group_sum(int e) {
@@ -1438,6 +1446,47 @@ struct Rows {
   static __device__ __forceinline__ int rep_at(RowT b1, int hbe, int x) {
     return x >= hbe ? 0 : hi(b1 & ((RowT(2) << x) - 1));
   }
+  static __device__ __forceinline__ int popc(RowT v) {
+    if constexpr (sizeof(RowT) == 4) return __popc(v);
+    else return __popcll((unsigned long long)v);
+  }
+  // runs of a periodic row with boundaries b (one for a constant row)
+  static __device__ __forceinline__ int runs(RowT b) { return b ? popc(b) : 1; }
+  // rank of the run holding x among the row's runs ordered by their first
+  // sites, the run that wraps past L-1 (first site 0) being rank 0; b1 and
+  // hbe as for rep_at
+  static __device__ __forceinline__ int rank(RowT b1, int hbe, int x) {
+    return x >= hbe ? 0 : popc(b1 & ((RowT(2) << x) - 1)) - 1;
+  }
+  // first site of the run of rank k
+  static __device__ __forceinline__ int run_first(RowT b, int k) {
+    RowT b1 = b | 1;
+    for (int i = 0; i < k; ++i) b1 &= b1 - 1;
+    return lo(b1);
+  }
+  // fn(k, first, seg) for every run of the row, seg being its sites
+  template <typename F>
+  static __device__ __forceinline__ void for_runs(RowT b, int L, RowT full, F fn) {
+    if (b == 0) {
+      fn(0, 0, full);
+      return;
+    }
+    const int first = lo(b);
+    int k = first == 0 ? 0 : 1;
+    RowT bb = b;
+    while (bb) {
+      const int st = lo(bb);
+      bb &= bb - 1;
+      RowT seg = (bb ? (RowT(1) << lo(bb)) : RowT(0)) - (RowT(1) << st);
+      if (!bb) seg &= full;
+      if (!bb && first != 0) {  // the last run wraps into the row start
+        fn(0, 0, seg | ((RowT(1) << first) - 1));
+      } else {
+        fn(k, st, seg);
+        ++k;
+      }
+    }
+  }
 };
 
 // Cluster labelling of a slab of `planes` full z-planes in LDS, all threads
@@ -1545,7 +1594,8 @@ __device__ __forceinline__ uint64_t row_bits(const Lattice &g, int r, Load load)
 // Move bits of word (p, h) gathered from the per-row move masks mrow; 4
 // threads per word, thread `quarter` gathering the 16 copies of z block
 // bz = quarter.
-__device__ __forceinline__ uint64_t word_move_bits(const Lattice &g, const uint64_t *mrow,
+template <typename RowT>
+__device__ __forceinline__ uint64_t word_move_bits(const Lattice &g, const RowT *mrow,
                                                    int p, int h, int quarter) {
   const int gz = qdiv(h, g.mhs), rem = h - gz * g.hs, yb = qdiv(rem, g.mhl);
   const int ax = 2 * (rem - yb * g.hl) + ((p + yb + gz) & 1);
@@ -1561,6 +1611,148 @@ __device__ __forceinline__ uint64_t word_move_bits(const Lattice &g, const uint6
   return m;
 }
 
+// The run forest: entry i holds the parent of run node i in its high 16
+// bits and the node's own first site (its smallest site index) in the low
+// 16 bits, which every write keeps, so that an atomic minimum on an entry
+// compares parents.  Loads and stores are relaxed workgroup-scope atomics,
+// so every step re-reads LDS.
+__device__ __forceinline__ uint32_t run_entry(const uint32_t *p, uint32_t i) {
+  return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Root of run node a with path splitting (each node passed is pointed at
+// its grandparent, an ancestor, so the trees keep their members).  Returns
+// the root's entry: its number in the high bits, its first site in the low.
+__device__ __forceinline__ uint32_t run_root(uint32_t *p, uint32_t a) {
+  uint32_t ea = run_entry(p, a);
+  while ((ea >> 16) != a) {
+    const uint32_t q = ea >> 16, eq = run_entry(p, q);
+    if ((eq >> 16) != q)
+      __hip_atomic_store(p + a, (eq & 0xFFFF0000u) | (ea & 0xFFFFu), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    a = q;
+    ea = eq;
+  }
+  return ea;
+}
+
+// Union of the trees of run nodes a and b: Rem's algorithm with splicing
+// (Patwary, Blair and Manne, 2010).  Every parent is at most its node, and
+// parents only ever decrease, so a tree's root is its smallest node.  Of
+// the two current nodes, the one with the larger parent takes the other's
+// parent by an atomic minimum: on a root that links the two trees; on any
+// other node it moves that node's subtree into the other tree (a splice),
+// and the walk goes on from the node's old parent.  It ends when both nodes
+// have one parent, or at a link; concurrent unions only lower parents
+// further, so every node stays connected to what it was joined with.
+__device__ __forceinline__ void rem_unite(uint32_t *p, uint32_t a, uint32_t b) {
+  for (;;) {
+    uint32_t ea = run_entry(p, a), eb = run_entry(p, b);
+    if ((ea >> 16) == (eb >> 16)) return;
+    if (ea < eb) {  // the larger parent is a's
+      const uint32_t t = a, te = ea;
+      a = b;
+      ea = eb;
+      b = t;
+      eb = te;
+    }
+    const uint32_t old = __hip_atomic_fetch_min(p + a, (eb & 0xFFFF0000u) | (ea & 0xFFFFu),
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if ((old >> 16) == a) return;  // a was a root: linked
+    a = old >> 16;
+  }
+}
+
+// The unions of a whole tube's labelling: for every row r, the y and z bonds
+// where a run overlap with the next row starts (when the bond at x-1 joins
+// the same two runs the union is implied), between the nodes of the runs
+// they join.  A lane finds the bonds of its rows and pushes the node pairs
+// onto its wave's queue (kHouQueue entries of two 16-bit nodes); once 64
+// are queued the wave unites them, one per lane, so every lane stays busy
+// however unevenly the bonds fall on the rows.  BY_RUNS: nodes are run
+// numbers (the first run number of row r, rbase[r], plus the run's rank);
+// otherwise sites (the run's first site, Rows::rep_at).  Lanes take rows
+// nrows/64 apart (spread_row), so concurrent unions mostly meet different
+// clusters.
+template <bool BY_RUNS, typename RowT, typename Unite>
+__device__ __forceinline__ void queue_links(const Lattice &g, const RowT *rows, RowT full,
+                                            const uint16_t *rbase, uint32_t *queue,
+                                            Unite unite) {
+  using R = Rows<RowT>;
+  const int L = g.L, nrows = g.V / L, T = blockDim.x, tid = threadIdx.x, lane = tid & 63;
+  uint32_t *wq = queue + (tid >> 6) * kHouQueue;
+  int qn = 0;  // pairs in the wave's queue, the same in every lane
+  auto drain = [&](int cnt) {
+    lds_wave_sync();  // the queue's writes are done
+    if (lane < cnt) {
+      const uint32_t v = wq[qn - cnt + lane];
+      unite(v & 0xFFFFu, v >> 16);
+    }
+    qn -= cnt;
+    lds_wave_sync();  // and read before the slots are written again
+  };
+  auto node_base = [&](int r) { return BY_RUNS ? (uint32_t)rbase[r] : (uint32_t)(r * L); };
+  auto node = [&](uint32_t nb, RowT b1, int hbe, int x) {
+    return nb + (uint32_t)(BY_RUNS ? R::rank(b1, hbe, x) : R::rep_at(b1, hbe, x));
+  };
+  for (int base = tid - lane; base < nrows; base += T) {
+    const int rr = base + lane;
+    RowT ty = 0, tz = 0, b1 = 0, by1 = 0, bz1 = 0;
+    int hbe = L, hby = L, hbz = L;
+    uint32_t n0 = 0, ny = 0, nz = 0;
+    if (rr < nrows) {
+      const int r = spread_row(rr, nrows);
+      const RowT q = rows[r], b = R::bounds(q, L, full);
+      const RowT lq = ~(q ^ (q << 1)) & full & ~RowT(1);  // q[x] == q[x-1]
+      b1 = b | 1;
+      hbe = R::wrap_start(b, L);
+      n0 = node_base(r);
+      const int zl = qdiv(r, g.mL), y = r - zl * L;
+      // where a run overlap with row r2 starts, and r2's constants
+      auto todo = [&](int r2, RowT &b21, int &hbe2) {
+        const RowT q2 = rows[r2], b2 = R::bounds(q2, L, full);
+        const RowT l2 = ~(q2 ^ (q2 << 1)) & full & ~RowT(1);
+        b21 = b2 | 1;
+        hbe2 = R::wrap_start(b2, L);
+        return (RowT)(~(q ^ q2) & full & ~(lq & l2));
+      };
+      const int ry = y + 1 == L ? r - (L - 1) : r + 1;
+      ty = todo(ry, by1, hby);
+      ny = node_base(ry);
+      int rz = -1;
+      if (zl + 1 < g.Lz)
+        rz = r + L;
+      else if (g.bc == EA_PBC)
+        rz = r - L * (g.Lz - 1);
+      if (rz >= 0) {
+        tz = todo(rz, bz1, hbz);
+        nz = node_base(rz);
+      }
+    }
+    for (;;) {
+      const bool has = (ty | tz) != 0;
+      const uint64_t m = __ballot(has);
+      if (!m) break;
+      if (has) {
+        const bool yb = ty != 0;
+        const int x = R::lo(yb ? ty : tz);
+        const uint32_t p0 = node(n0, b1, hbe, x);
+        const uint32_t p1 = yb ? node(ny, by1, hby, x) : node(nz, bz1, hbz, x);
+        if (yb)
+          ty &= ty - 1;
+        else
+          tz &= tz - 1;
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        wq[qn + (int)below] = p0 | p1 << 16;
+      }
+      qn += __popcll(m);
+      if (qn >= 64) drain(64);
+    }
+  }
+  while (qn > 0) drain(qn < 64 ? qn : 64);
+}
+
 // Whole-tube Houdayer move of one instance (group, sample, beta < ctm) in a
 // single workgroup: overlap q = s1 ^ s2 of replicas r1, r2 at beta
 // (d_computeoverlap), clusters of equal q labelled in LDS
@@ -1569,14 +1761,25 @@ __device__ __forceinline__ uint64_t word_move_bits(const Lattice &g, const uint6
 // (spin_select, :358-374), then q' = q ^ coin(label) and s1 ^= q', s2 ^= q'
 // (spin_flip_YK_custom + d_pack + d_useoverlap).
 //
-// The forest has one node per site, but only the first site of each run of
-// equal q along x (its representative, Rows::rep) takes part: a bond is
-// united between the representatives of its two ends, and each run reads
-// its cluster's root once.  A row's move mask is then built run by run.
+// The nodes of the labelling are the runs of equal q along x.  They are
+// numbered in site order -- row by row (row r = y + L z), and within a row
+// in the order of their first sites, the run that wraps past x = L-1
+// belonging to the row start -- so the smallest run number of a cluster is
+// the run holding its smallest site.  A forest of 32-bit parents over the
+// run numbers is linked with Rem's union with splicing (parents only ever
+// decrease, so a root is its tree's smallest node; the atomic minimum on a
+// root links it, on any other node it splices that node's subtree into the
+// other tree, and the walk goes on from its old parent until both sides
+// meet); the unions come from a per-wave queue (queue_links).  Then every
+// root run draws the coin of its first site, every run takes its root's
+// coin as its move bit, and each row's move mask is built site by site
+// from its runs' bits -- passes whose lanes all take the same number of
+// steps, unlike loops over the runs of a row.  An instance with more runs
+// than the run forest holds (hou_cap, near the largest possible count of V
+// runs) is labelled with a forest over sites instead (16-bit parents,
+// Komura's union), as a whole tube whose runs fit always is.
 // The kernel also recomputes the energies of the two configurations it
 // changed (the Measure_Energy that follows a cluster move, main.cu:209).
-// Dynamic LDS: rows[V/L] u64 | qw[2n] u64 | nw[2n] u64 | cb[V/32] u32 (coin
-// of every site) | forest[V] u32.
 template <typename RowT, typename IdxT>
 __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map,
                                                     Phase ph) {
@@ -1589,12 +1792,20 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   const auto spin = gmem(a.spin);
   const Inst in = instance_of(a, wh.local, wh.step, ph.call);
   const int V = g.V, L = g.L, nrows = V / L, T = blockDim.x, tid = threadIdx.x;
-  uint64_t *rows = lds;         // q rows, then move rows
-  uint64_t *qw = rows + nrows;  // q words, then replica r1's new words
-  uint64_t *nw = qw + 2 * g.n;  // replica r2's new words
-  uint32_t *cb = reinterpret_cast<uint32_t *>(nw + 2 * g.n);
-  const Forest<IdxT, true> f{reinterpret_cast<IdxT *>(cb + V / 32)};
+  const int cap = a.hou_cap;
+  const HouWholeLds lay = hou_whole_layout(g, cap, T, sizeof(RowT), sizeof(IdxT));
+  uint8_t *base = reinterpret_cast<uint8_t *>(lds);
+  RowT *rows = reinterpret_cast<RowT *>(base);  // q rows, then move rows
+  uint64_t *qw = reinterpret_cast<uint64_t *>(base + lay.qw);  // q words, then r1's new words
+  uint32_t *cb = reinterpret_cast<uint32_t *>(base + lay.cb);  // coin of every site
+  uint16_t *rbase = reinterpret_cast<uint16_t *>(base + lay.rbase);  // first run of a row
+  uint32_t *rp = reinterpret_cast<uint32_t *>(base + lay.forest);    // run forest
+  const Forest<IdxT, true> f{reinterpret_cast<IdxT *>(base + lay.forest)};  // site forest
+  uint32_t *queue = reinterpret_cast<uint32_t *>(base + lay.region);  // while linking
+  uint64_t *nw = reinterpret_cast<uint64_t *>(base + lay.region);     // r2's new words
   __shared__ int esum[2];
+  __shared__ int wtot[16];
+  __shared__ int nruns;
   const int beta = in.beta, s = in.s, grp = in.grp;
   const Pair pw = pair_words(g, a, in.pairs, s, grp, beta);
   if (tid < 2) esum[tid] = 0;
@@ -1603,37 +1814,8 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
     const size_t o = (size_t)p * g.half + h;
     qw[w] = spin[pw.o1 + o] ^ spin[pw.o2 + o];
   }
-  for (int i = tid; i < V; i += T) f.p[i] = f.ix(i);  // every entry its own root
-  __syncthreads();
-  for (int r = tid; r < nrows; r += T)
-    rows[r] = row_bits(g, r, [&](int p, int h) { return qw[p * g.n + h]; });
-  __syncthreads();
-
-  const RowT full = (RowT)(L == 64 ? ~0ULL : ((1ULL << L) - 1));
-  for (int rr = tid; rr < nrows; rr += T) {
-    const int r = spread_row(rr, nrows);
-    const RowT q = (RowT)rows[r], b = R::bounds(q, L, full), b1 = b | 1;
-    const int hbe = R::wrap_start(b, L);
-    const RowT lq = ~(q ^ (q << 1)) & full & ~RowT(1);  // q[x] == q[x-1]
-    const int zl = qdiv(r, g.mL), y = r - zl * L;
-    auto link = [&](int r2) {
-      const RowT q2 = (RowT)rows[r2], b2 = R::bounds(q2, L, full), b21 = b2 | 1;
-      const int hbe2 = R::wrap_start(b2, L);
-      const RowT l2 = ~(q2 ^ (q2 << 1)) & full & ~RowT(1);
-      RowT todo = ~(q ^ q2) & full & ~(lq & l2);  // where a run overlap starts
-      while (todo) {
-        const int x = R::lo(todo);
-        todo &= todo - 1;
-        f.unite(r * L + R::rep_at(b1, hbe, x), r2 * L + R::rep_at(b21, hbe2, x));
-      }
-    };
-    link(y + 1 == L ? r - (L - 1) : r + 1);
-    if (zl + 1 < g.Lz)
-      link(r + L);
-    else if (g.bc == EA_PBC)
-      link(r - L * (g.Lz - 1));
-  }
-  // coins of 128 consecutive sites per Philox call
+  // coins of 128 consecutive sites per Philox call (they do not depend on
+  // the clusters)
   const uint64_t cidx = in.coin + (uint64_t)s * V * g.nt + (uint64_t)beta * V;
   for (int c = tid; c < V / 128; c += T) {
     const uint64_t blk = (cidx >> 7) + c;
@@ -1645,38 +1827,120 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
     cb[4 * c + 3] = v3;
   }
   __syncthreads();
-  if (a.keep) {
-    const auto out = gv(a.keep) + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
-    for (int i = tid; i < V; i += T) {
-      const int r = qdiv(i, g.mL), x = i - r * L;
-      out[i] = f.root(r * L + R::rep(R::bounds((RowT)rows[r], L, full), x));
+  const RowT full = (RowT)(L == 64 ? ~0ULL : ((1ULL << L) - 1));
+  // the q rows, each thread a contiguous range of them, and the run numbers:
+  // rbase[r] = runs of the rows before r (a block-wide scan)
+  {
+    const int per = (nrows + T - 1) / T, r0 = min(tid * per, nrows), r1 = min(r0 + per, nrows);
+    int own = 0;
+    for (int r = r0; r < r1; ++r) {
+      const RowT q = (RowT)row_bits(g, r, [&](int p, int h) { return qw[p * g.n + h]; });
+      rows[r] = q;
+      own += R::runs(R::bounds(q, L, full));
     }
-  }
-  for (int r = tid; r < nrows; r += T) {  // move mask of row r, run by run
-    const RowT q = (RowT)rows[r], b = R::bounds(q, L, full);
-    const int base = r * L;
-    RowT m = 0;
-    if (b == 0) {
-      const uint32_t root = f.root(base);
-      m = ((cb[root >> 5] >> (root & 31)) & 1u) ? full : 0;
-    } else {
-      const int first = R::lo(b);
-      RowT bb = b;
-      while (bb) {
-        const int st = R::lo(bb);
-        bb &= bb - 1;
-        RowT seg = (bb ? (RowT(1) << R::lo(bb)) : RowT(0)) - (RowT(1) << st);
-        if (!bb) seg &= full;
-        int rp = st;
-        if (!bb && first != 0) {  // the last run wraps into the row start
-          rp = 0;
-          seg |= (RowT(1) << first) - 1;
+    const int lane = tid & 63;
+    int x = own;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wtot[tid >> 6] = x;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < T / 64; ++w) {
+      before += w < (tid >> 6) ? wtot[w] : 0;
+      total += wtot[w];
+    }
+    int run = before + x - own;  // runs of the rows before r0
+    for (int r = r0; r < r1; ++r) {
+      rbase[r] = (uint16_t)run;
+      const RowT b = R::bounds(rows[r], L, full);
+      if (total <= cap) {
+        // every run its own root; its first sites: bit 0 (rank 0, the run
+        // that wraps included) and the boundaries but the wrapping one
+        RowT st = b | 1;
+        const int hbe = R::wrap_start(b, L);
+        if (hbe < L) st &= ~(RowT(1) << hbe);
+        for (uint32_t id = run; st; ++id) {
+          rp[id] = id << 16 | (uint32_t)(r * L + R::lo(st));
+          st &= st - 1;
         }
-        const uint32_t root = f.root(base + rp);
-        if ((cb[root >> 5] >> (root & 31)) & 1u) m |= seg;
+      }
+      run += R::runs(b);
+    }
+    if (tid == T - 1) nruns = run;
+  }
+  __syncthreads();
+  const int nrun = nruns;
+  const bool by_runs = nrun <= cap;
+
+  if (by_runs) {
+    queue_links<true>(g, rows, full, rbase, queue,
+                      [&](uint32_t u, uint32_t v) { rem_unite(rp, u, v); });
+    __syncthreads();
+    if (a.keep) {
+      const auto out = gv(a.keep) + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
+      for (int i = tid; i < V; i += T) {
+        const int r = qdiv(i, g.mL), x = i - r * L;
+        const RowT b = R::bounds(rows[r], L, full);
+        out[i] = run_root(rp, rbase[r] + R::rank(b | 1, R::wrap_start(b, L), x)) & 0xFFFFu;
+      }
+      __syncthreads();  // the q rows are read before they become move rows
+    }
+    // the move bit of every run is the coin of its root's first site, 64
+    // runs per wave step (the queues are done with, so the bits take their
+    // place)
+    uint64_t *mv = reinterpret_cast<uint64_t *>(queue);
+    {
+      const int lane = tid & 63;
+      for (int b0 = tid - lane; b0 < nrun; b0 += T) {
+        const int id = b0 + lane;
+        bool bit = false;
+        if (id < nrun) {
+          const uint32_t site = run_root(rp, id) & 0xFFFFu;
+          bit = (cb[site >> 5] >> (site & 31)) & 1u;
+        }
+        const uint64_t m = __ballot(bit);
+        if (lane == 0) mv[b0 >> 6] = m;
       }
     }
-    rows[r] = m;  // only this thread reads row r in this loop
+    __syncthreads();
+    // the move mask of each row: every site takes its run's move bit
+    for (int r = tid; r < nrows; r += T) {
+      const RowT b = R::bounds(rows[r], L, full), b1 = b | 1;
+      const int hbe = R::wrap_start(b, L);
+      const uint32_t id0 = rbase[r], sh = id0 & 63;
+      const int nr_ = R::runs(b);
+      const uint64_t w0 = mv[id0 >> 6];
+      const uint64_t w1 = sh + nr_ > 64 ? mv[(id0 >> 6) + 1] : 0;
+      const uint64_t rm = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;  // bit k: run k moves
+      RowT m = 0;
+      for (int x = 0; x < L; ++x) m |= (RowT)((rm >> R::rank(b1, hbe, x)) & 1u) << x;
+      rows[r] = m;  // only this thread reads row r in this loop
+    }
+  } else {
+    for (int i = tid; i < V; i += T) f.p[i] = f.ix(i);  // every site its own root
+    __syncthreads();
+    queue_links<false>(g, rows, full, rbase, queue,
+                       [&](uint32_t u, uint32_t v) { f.unite(u, v); });
+    __syncthreads();
+    if (a.keep) {
+      const auto out = gv(a.keep) + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
+      for (int i = tid; i < V; i += T) {
+        const int r = qdiv(i, g.mL), x = i - r * L;
+        out[i] = f.root(r * L + R::rep(R::bounds(rows[r], L, full), x));
+      }
+      __syncthreads();  // the q rows are read before they become move rows
+    }
+    for (int r = tid; r < nrows; r += T) {  // move mask of row r, run by run
+      RowT m = 0;
+      R::for_runs(R::bounds(rows[r], L, full), L, full, [&](int, int first, RowT seg) {
+        const uint32_t root = f.root(r * L + first);
+        if ((cb[root >> 5] >> (root & 31)) & 1u) m |= seg;
+      });
+      rows[r] = m;  // only this thread reads row r in this loop
+    }
   }
   __syncthreads();
   for (int t0 = 0; t0 < 8 * g.n; t0 += T) {  // every lane reaches the shuffles
@@ -2137,9 +2401,13 @@ size_t energy_partials(const Lattice &g) {
 // 16-bit parents when every site index fits (and the rows are 32 bits).
 static bool hou_idx16(const Lattice &g) { return g.L <= 32 && g.V <= 65536; }
 
-size_t hou_whole_lds(const Lattice &g) {
-  return 8 * (size_t)(g.V / g.L) + 32 * (size_t)g.n + 4 * (size_t)(g.V / 32) +
-         (hou_idx16(g) ? 2 : 4) * (size_t)g.V;
+int hou_run_cap(const Lattice &g) {
+  if (g.V <= 8192) return g.V;
+  return std::min((g.V * 9 / 16 + 63) / 64 * 64, 65535);
+}
+
+size_t hou_whole_lds(const Lattice &g, int threads, int cap) {
+  return hou_whole_layout(g, cap, threads, g.L <= 32 ? 4 : 8, hou_idx16(g) ? 2 : 4).total;
 }
 
 size_t hou_slab_lds(const Lattice &g, int planes) {
@@ -2152,7 +2420,8 @@ size_t hou_apply_lds(const Lattice &g, int planes) {
 }
 
 bool hou_use_slabs(const Lattice &g, int lds_max, int lds_cu, int cus) {
-  const size_t whole = hou_whole_lds(g);
+  if (g.V > 65536) return true;  // the whole-tube queues hold 16-bit node pairs
+  const size_t whole = hou_whole_lds(g, 1024, hou_run_cap(g));
   if (whole > (size_t)lds_max) return true;
   return whole > (size_t)lds_cu / 2 && 2LL * g.ns * g.ctm < cus;
 }
@@ -2195,13 +2464,13 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
     if (t.planes == 0) {
       if (hou_idx16(g)) {
         add(kMapWhole16, inst);
-        p.lds16 = std::max(p.lds16, hou_whole_lds(g));
+        p.lds16 = std::max(p.lds16, hou_whole_lds(g, shape.cluster_threads, t.hou_cap));
       } else if (g.L <= 32) {
         add(kMapWhole32, inst);
-        p.lds32 = std::max(p.lds32, hou_whole_lds(g));
+        p.lds32 = std::max(p.lds32, hou_whole_lds(g, shape.cluster_threads, t.hou_cap));
       } else {
         add(kMapWhole64, inst);
-        p.lds64 = std::max(p.lds64, hou_whole_lds(g));
+        p.lds64 = std::max(p.lds64, hou_whole_lds(g, shape.cluster_threads, t.hou_cap));
       }
     } else {
       const long long slabs = (g.Lz + t.planes - 1) / t.planes;

@@ -112,6 +112,7 @@ struct Tube {
   uint64_t *hrow, *hflag;
   uint32_t *keep;           // copy of the last labels, or NULL
   int planes;               // slab Houdayer planes; 0: whole tube in one block
+  int hou_cap;              // run nodes of the whole-tube labelling (hou_run_cap)
   int nbx;                  // sweep blocks per beta
   int cluster;              // Houdayer moves are part of this tube's steps
   int pt_exact;             // every PT walk applies the exact rule pair by pair
@@ -218,7 +219,36 @@ hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
 // LDS bytes of the whole-tube and slab Houdayer kernels; a tube is labelled
 // in one workgroup or in slabs as hou_use_slabs decides.  Both need
 // L <= 64.
-size_t hou_whole_lds(const Lattice &g);
+constexpr int kHouQueue = 128;  // union queue entries per wave (whole-tube kernel)
+// Run nodes the whole-tube labelling holds (Tube::hou_cap): every run of a
+// small tube, about 9/16 of the sites otherwise (a tube of uncorrelated q
+// has about V/2 runs); an instance with more runs is labelled over sites.
+int hou_run_cap(const Lattice &g);
+// Byte offsets of the whole-tube kernel's dynamic LDS with `threads`
+// threads, run capacity cap, rows of rowb bytes and site-forest parents of
+// idxb bytes: q rows | q words | coin bits | first run of each row (u16) |
+// the run forest (u32) or the site forest | the waves' union queues, then
+// the runs' move bits, then replica r2's new words.
+struct HouWholeLds {
+  uint32_t qw, cb, rbase, forest, region, total;
+};
+__host__ __device__ inline uint32_t hou_up16(uint32_t x) { return (x + 15u) & ~15u; }
+__host__ __device__ inline HouWholeLds hou_whole_layout(const Lattice &g, int cap, int threads,
+                                                         int rowb, int idxb) {
+  const uint32_t nrows = (uint32_t)(g.V / g.L);
+  const uint32_t forest = (uint32_t)(4 * cap) > (uint32_t)(idxb * g.V) ? (uint32_t)(4 * cap)
+                                                                        : (uint32_t)(idxb * g.V);
+  const uint32_t queues = (uint32_t)(threads / 64) * 4u * (uint32_t)kHouQueue;
+  HouWholeLds l;
+  l.qw = hou_up16(nrows * (uint32_t)rowb);
+  l.cb = l.qw + 16u * (uint32_t)g.n;
+  l.rbase = l.cb + (uint32_t)g.V / 8u;
+  l.forest = l.rbase + hou_up16(2u * nrows);
+  l.region = l.forest + hou_up16(forest);
+  l.total = l.region + (16u * (uint32_t)g.n > queues ? 16u * (uint32_t)g.n : queues);
+  return l;
+}
+size_t hou_whole_lds(const Lattice &g, int threads, int cap);
 size_t hou_slab_lds(const Lattice &g, int planes);
 size_t hou_apply_lds(const Lattice &g, int planes);
 constexpr int kHouSlabSites = 65536;  // slab-local labels are 16-bit

@@ -362,6 +362,27 @@ def test_exact_pt_walk_matches_oracle(case, monkeypatch):
         assert_same_state(o, e, "exact walk, after %d steps" % chunk)
 
 
+@pytest.mark.parametrize("case", [
+    dict(L=8, Lz=16, nr=4, nt=6, ns=3, bc=1),
+    dict(L=16, Lz=64, nr=4, nt=24, ns=2, bc=1, tmin=0.6, tmax=1.6),
+    dict(L=24, Lz=48, nr=4, nt=4, ns=2, bc=0),
+])
+def test_site_forest_labelling_matches_oracle(case, monkeypatch):
+    """Whole-tube labelling forced onto its forest over sites (the path an
+    instance with more runs than the run forest holds takes): the same
+    labels, moves and energies as the oracle's."""
+    monkeypatch.setenv("EA_HOU_SITE_FOREST", "1")
+    o, e = pair(**case)
+    o.keep_labels()
+    e.engine.keep_labels()
+    for chunk in (1, 2):
+        e.steps(chunk, cluster=True)
+        o.steps(chunk, cluster=True)
+        assert_same_state(o, e, "site forest, after %d steps" % chunk)
+        assert np.array_equal(e.engine.download_labels().ravel(),
+                              o.field("last_labels")), "labels"
+
+
 @pytest.mark.parametrize("L,Lz,nr,nt,ns,bc", [(8, 16, 4, 6, 3, 1), (16, 32, 2, 5, 2, 0),
                                                (16, 264, 4, 3, 1, 0)])
 def test_overlaps_match_oracle(L, Lz, nr, nt, ns, bc):
