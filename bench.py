@@ -194,7 +194,7 @@ def batched_compare(steps, samples=1, ntubes=48):
             "unit": "spin-flip attempts/ns"}
 
 
-def serving(nreq, rate, sweeps, budget_gb=16.0, chunk=32):
+def serving(nreq, rate, sweeps, budget_gb=64.0, chunk=128):
     """Poisson stream of simulation requests through the continuous-batching
     server: p50/p99 completion latency (arrival to completion) and
     whole-run throughput."""
@@ -216,8 +216,9 @@ def serving(nreq, rate, sweeps, budget_gb=16.0, chunk=32):
                                  if k.startswith("t_")},
                 "request_shape": "L in {8,16,24,32}, Lz multiple of 8 in "
                                  "[L,16L], PBC|OBC, NT in {16,24,32} in "
-                                 "[0.6,1.6], 4 replicas, 1 sample, sweep+PT "
-                                 "per step"})
+                                 "[0.6,1.6], 4 replicas, 1 sample, sweeps "
+                                 "log-uniform in [%d,%d], sweep+PT+Houdayer per "
+                                 "step" % sweeps})
     return out
 
 
@@ -254,6 +255,82 @@ def cpu_baseline(w, S, budget_s):
                       % (n_all, S, dt_all, cores, n_one, dt_one)}
 
 
+def measure(name, S, steps, warmup, timing_steps, tune=True, shape=None, tune_steps=50):
+    """One workload at S samples: the launch shape (tuned, or given as
+    "P,C,K,X"), `warmup` untimed steps, `steps` timed steps of the engine
+    with its inputs resident (the host counter blocks and replica pairs are
+    made inside the timed region), then per-kernel durations from HIP
+    events over `timing_steps` steps and each kernel's algorithmic GB/s."""
+    w = WORKLOADS[name]
+    V = w["L"] * w["L"] * w["Lz"]
+    betas = eatube.betas_linear(w["tmin"], w["tmax"], w["nt"])
+    sim = eatube.Simulation(w["L"], w["Lz"], w["nr"], w["nt"], S, bc=w["bc"],
+                            betas=betas, seeds=SEEDS)
+    eng = sim.engine
+    ctm = eng.ctm if w["cluster"] else 0
+    shp, shape_times = eatube.DEFAULT_SHAPE, {}
+    if shape:
+        shp = tuple(int(x) for x in shape.split(","))
+    elif tune:
+        shp, shape_times = eatube.tune_shape([eng], cluster=w["cluster"], steps=tune_steps)
+        log(name, "launch shape", shp, {k: round(v * 1e3 / tune_steps, 4)
+                                         for k, v in shape_times.items()})
+    eng.set_shape(*shp)
+    log(name, "warmup", warmup)
+    sim.steps(warmup, cluster=w["cluster"])
+    eng.sync()
+    log(name, "timed", steps)
+    t0 = time.perf_counter()
+    sch = sim.schedule(steps, cluster=w["cluster"])
+    eng.run(sch)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    value = steps * V * w["nr"] * w["nt"] * S / (wall * 1e9)
+    # per-kernel durations, HIP events on the engine's stream
+    eng.set_timing(True)
+    sim.steps(timing_steps, cluster=w["cluster"])
+    tm = eng.read_timing()
+    eng.set_timing(False)
+    kern = {}
+    for k, (ms, n) in tm.items():
+        if n:
+            b = kernel_bytes(k, w, S, V, ctm)
+            kern[k] = {"avg_us": 1e3 * ms / n, "launches": n,
+                       "share": 0.0, "GBs": b / (1e-3 * ms / n) / 1e9,
+                       "frac": b / (1e-3 * ms / n) / 1e9 / HBM_PEAK_GBS}
+    tot = sum(v["avg_us"] * v["launches"] for v in kern.values())
+    for v in kern.values():
+        v["share"] = v["avg_us"] * v["launches"] / tot
+    dom = max(kern, key=lambda k: kern[k]["share"])
+    ach = kern[dom]["GBs"]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": ach,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": measured_traffic(name, dom),
+            "bytes_per_launch": kernel_bytes(dom, w, S, V, ctm),
+            "avg_launch_us": kern[dom]["avg_us"]}
+    eng.close()
+    return {"workload": name, "value": value, "unit": "spin-flip attempts/ns",
+            "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * wall / steps,
+            "wall_s": wall, "samples": S, "houdayer_betas": ctm,
+            "shape": shp, "shape_times": shape_times, "kernels": kern, "roofline": roof}
+
+
+def sub_record(name, S, steps, warmup, timing_steps):
+    """A further configuration measured like the headline, in brief."""
+    r = measure(name, S, steps, warmup, timing_steps, tune_steps=max(2, min(50, steps)))
+    w = WORKLOADS[name]
+    return {"value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"],
+            "steps": steps, "warmup": warmup,
+            "config": {"L": w["L"], "Lz": w["Lz"], "bc": "OBC" if w["bc"] else "PBC",
+                       "replicas": w["nr"], "temperatures": w["nt"],
+                       "T_range": [w["tmin"], w["tmax"]], "samples": S,
+                       "houdayer": bool(w["cluster"]), "houdayer_betas": r["houdayer_betas"]},
+            "launch_shape": list(r["shape"]),
+            "kernels": {k: {"avg_us": v["avg_us"], "GBs": v["GBs"], "frac": v["frac"],
+                            "share": v["share"]} for k, v in r["kernels"].items()},
+            "roofline": r["roofline"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,9 +350,14 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=50)
     ap.add_argument("--batch-steps", type=int, default=200,
                     help="steps of the 48-tube mixed batch comparison (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=40,
+    ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving run (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=0.5,
+                    help="Poisson arrivals per second of the serving run")
+    ap.add_argument("--serve-chunk", type=int, default=128,
+                    help="steps every running request advances between admissions")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="skip the plain and large configurations' sub-records")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a MIB-MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -292,8 +374,6 @@ def main():
         copy_calibration(args.copy_calibration)
     w = WORKLOADS[args.workload]
     S = args.samples
-    V = w["L"] * w["L"] * w["Lz"]
-    betas = eatube.betas_linear(w["tmin"], w["tmax"], w["nt"])
 
     parity = None
     if not args.no_parity:
@@ -305,52 +385,11 @@ def main():
                                        "failed; no timing reported"}))
             sys.exit(1)
 
-    sim = eatube.Simulation(w["L"], w["Lz"], w["nr"], w["nt"], S, bc=w["bc"],
-                            betas=betas, seeds=SEEDS)
-    eng = sim.engine
-    ctm = eng.ctm
-    shape, shape_times = eatube.DEFAULT_SHAPE, {}
-    if args.shape:
-        shape = tuple(int(x) for x in args.shape.split(","))
-    elif not args.no_tune:
-        shape, shape_times = eatube.tune_shape([eng], cluster=w["cluster"],
-                                               steps=50)
-        log("launch shape", shape, {k: round(v * 20, 4)
-                                    for k, v in shape_times.items()})
-    eng.set_shape(*shape)
-    log("warmup", args.warmup)
-    sim.steps(args.warmup, cluster=w["cluster"])
-    eng.sync()
-    log("timed", args.steps)
-    t0 = time.perf_counter()
-    sch = sim.schedule(args.steps, cluster=w["cluster"])
-    eng.run(sch)
-    eng.sync()
-    wall = time.perf_counter() - t0
-    attempts = args.steps * V * w["nr"] * w["nt"] * S
-    value = attempts / (wall * 1e9)
-
-    # per-kernel durations, HIP events on the engine's stream
-    eng.set_timing(True)
-    sim.steps(args.timing_steps, cluster=w["cluster"])
-    tm = eng.read_timing()
-    eng.set_timing(False)
-    kern = {}
-    for k, (ms, n) in tm.items():
-        if n:
-            b = kernel_bytes(k, w, S, V, ctm)
-            kern[k] = {"avg_us": 1e3 * ms / n, "launches": n,
-                       "share": 0.0, "GBs": b / (1e-3 * ms / n) / 1e9}
-    tot = sum(v["avg_us"] * v["launches"] for v in kern.values())
-    for v in kern.values():
-        v["share"] = v["avg_us"] * v["launches"] / tot
-    dom = max(kern, key=lambda k: kern[k]["share"])
-    ach = kern[dom]["GBs"]
-    roof = {"bound": "hbm", "kernel": dom, "achieved": ach,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.workload, dom),
-            "bytes_per_launch": kernel_bytes(dom, w, S, V, ctm),
-            "avg_launch_us": kern[dom]["avg_us"]}
+    head = measure(args.workload, S, args.steps, args.warmup, args.timing_steps,
+                   tune=not args.no_tune, shape=args.shape)
+    shape, shape_times, kern, roof = (head["shape"], head["shape_times"], head["kernels"],
+                                      head["roofline"])
+    value, wall, ctm = head["value"], head["wall_s"], head["houdayer_betas"]
     sweep = kern.get("sweep")
     out = {
         "metric": "spin-flip attempts/ns",
@@ -385,17 +424,22 @@ def main():
         out["sweep_roofline"] = {"achieved": sweep["GBs"], "peak": HBM_PEAK_GBS,
                                  "frac": sweep["GBs"] / HBM_PEAK_GBS,
                                  "avg_launch_us": sweep["avg_us"]}
+    if not args.no_sub:
+        # the plain small tube (no cluster moves) and the large tube
+        log("sub-record plain_L8_Lz16")
+        out["plain_L8_Lz16"] = sub_record("plain_L8_Lz16", S, 2000, 100, 50)
+        log("sub-record large_L48_Lz384")
+        out["large_L48_Lz384"] = sub_record("large_L48_Lz384", S, 20, 3, 3)
     if args.batch_steps > 0:
         log("batched vs plain, 48 mixed tubes")
         out["batched"] = batched_compare(args.batch_steps)
     if args.serve_requests > 0:
         log("serving %d requests" % args.serve_requests)
         out["serving"] = serving(args.serve_requests, args.serve_rate,
-                                 (200, 2000))
+                                 (1000, 100000), chunk=args.serve_chunk)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(w, S, args.cpu_seconds)
-    eng.close()
     print(json.dumps(out))
 
 

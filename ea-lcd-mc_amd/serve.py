@@ -135,11 +135,16 @@ class Server:
     workers    threads preparing arrived requests on the host (prepare_fn)
                while the GPU serves; 0: each request is prepared inside
                start_fn at its admission
+    shape      launch shape of the batches (eatube.Batch.set_shape
+               arguments); the default suits requests of one sample: one
+               thread per position makes its random words and updates the
+               sample (sweep kind 1)
     """
 
     def __init__(self, budget, chunk=32, max_tubes=256, keep=False,
                  start_fn=start, batch_fn=eatube.Batch, footprint_fn=footprint,
-                 clock=time.perf_counter, workers=0, prepare_fn=prepare):
+                 clock=time.perf_counter, workers=0, prepare_fn=prepare,
+                 shape=(64, 512, 1, 0)):
         self.budget = int(budget)
         self.chunk = int(chunk)
         self.max_tubes = int(max_tubes)
@@ -147,6 +152,7 @@ class Server:
         self.start_fn, self.batch_fn = start_fn, batch_fn
         self.footprint_fn, self.clock = footprint_fn, clock
         self.workers, self.prepare_fn = int(workers), prepare_fn
+        self.shape = tuple(shape) if shape else None
         self.luts = _Luts()
         self.stats = dict(chunks=0, batches_built=0, max_running=0,
                           max_bytes=0, deferred=0, t_start=0.0, t_batch=0.0,
@@ -237,6 +243,8 @@ class Server:
                         batch.close()
                         batch = None
                     batch = self.batch_fn([running[k][1].engine for k in order])
+                    if self.shape and hasattr(batch, "set_shape"):
+                        batch.set_shape(*self.shape)
                     members = order
                     self.stats["batches_built"] += 1
                 tb = self.clock()
