@@ -1647,18 +1647,15 @@ __device__ __forceinline__ uint32_t run_root(uint32_t *p, uint32_t a) {
 // further, so every node stays connected to what it was joined with.
 __device__ __forceinline__ void rem_unite(uint32_t *p, uint32_t a, uint32_t b) {
   for (;;) {
-    uint32_t ea = run_entry(p, a), eb = run_entry(p, b);
-    if ((ea >> 16) == (eb >> 16)) return;
-    if (ea < eb) {  // the larger parent is a's
-      const uint32_t t = a, te = ea;
-      a = b;
-      ea = eb;
-      b = t;
-      eb = te;
-    }
-    const uint32_t old = __hip_atomic_fetch_min(p + a, (eb & 0xFFFF0000u) | (ea & 0xFFFFu),
+    const uint32_t ea = run_entry(p, a), eb = run_entry(p, b);
+    if ((ea ^ eb) < 0x10000u) return;  // one parent
+    // x: the node with the larger parent, y the other (selects, no branch)
+    const bool sw = ea < eb;
+    const uint32_t x = sw ? b : a, ex = sw ? eb : ea, ey = sw ? ea : eb;
+    b = sw ? a : b;
+    const uint32_t old = __hip_atomic_fetch_min(p + x, (ey & 0xFFFF0000u) | (ex & 0xFFFFu),
                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if ((old >> 16) == a) return;  // a was a root: linked
+    if ((old >> 16) == x) return;  // x was a root: linked
     a = old >> 16;
   }
 }
@@ -1807,13 +1804,30 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   __shared__ int wtot[16];
   __shared__ int nruns;
   const int beta = in.beta, s = in.s, grp = in.grp;
-  const Pair pw = pair_words(g, a, in.pairs, s, grp, beta);
-  if (tid < 2) esum[tid] = 0;
-  for (int w = tid; w < 2 * g.n; w += T) {
-    const int p = w >= g.n, h = w - p * g.n;
-    const size_t o = (size_t)p * g.half + h;
-    qw[w] = spin[pw.o1 + o] ^ spin[pw.o2 + o];
+  // The instance's two chains: replicas r1, r2 of the call's pairs and their
+  // clones at beta (the clones of all replicas are loaded at once, one per
+  // lane, beside the pairs), then each chain's energy before the move and
+  // its running sum at beta, read here so that the block's end only stores.
+  const int lane = tid & 63;
+  const int clone = gv(a.wctb)[((size_t)s * g.nr + min(lane, g.nr - 1)) * g.nt + beta];
+  const int r1 = gv(in.pairs)[2 * grp], r2 = gv(in.pairs)[2 * grp + 1];
+  const int c1 = __builtin_amdgcn_readlane(clone, r1), c2 = __builtin_amdgcn_readlane(clone, r2);
+  const size_t ch1 = (size_t)s * g.nr + r1, ch2 = (size_t)s * g.nr + r2;
+  const size_t sb = (size_t)s * 2 * g.half;
+  const size_t o1 = sb + spin_row(g, r1, c1), o2 = sb + spin_row(g, r2, c2);
+  int e_old = 0;
+  double e_sum = 0.0;
+  if (tid < 2) {
+    const size_t ch = tid ? ch2 : ch1;
+    e_old = gv(a.ener)[ch * g.nt + (tid ? c2 : c1)];
+    if (ph.measure) e_sum = gv(a.esum)[ch * g.nt + beta];
+    esum[tid] = 0;
   }
+  // the q words; the first of every thread is loaded (at a clamped index)
+  // before the coins are drawn, so that its latency overlaps them
+  const int w0 = min(tid, 2 * g.n - 1);
+  const size_t oo = (size_t)(w0 >= g.n) * g.half + (w0 - (w0 >= g.n) * g.n);
+  const uint64_t x1 = spin[o1 + oo], x2 = spin[o2 + oo];
   // coins of 128 consecutive sites per Philox call (they do not depend on
   // the clusters)
   const uint64_t cidx = in.coin + (uint64_t)s * V * g.nt + (uint64_t)beta * V;
@@ -1825,6 +1839,12 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
     cb[4 * c + 1] = v1;
     cb[4 * c + 2] = v2;
     cb[4 * c + 3] = v3;
+  }
+  if (tid < 2 * g.n) qw[tid] = x1 ^ x2;
+  for (int w = tid + T; w < 2 * g.n; w += T) {
+    const int p = w >= g.n, h = w - p * g.n;
+    const size_t o = (size_t)p * g.half + h;
+    qw[w] = spin[o1 + o] ^ spin[o2 + o];
   }
   __syncthreads();
   const RowT full = (RowT)(L == 64 ? ~0ULL : ((1ULL << L) - 1));
@@ -1954,9 +1974,9 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
     if (live && quarter == 0) {
       const size_t o = (size_t)p * g.half + h;
       const uint64_t qn = qw[w] ^ m;  // q' = q ^ coin(label)
-      const uint64_t x1 = spin[pw.o1 + o] ^ qn, x2 = spin[pw.o2 + o] ^ qn;
-      spin[pw.o1 + o] = x1;
-      spin[pw.o2 + o] = x2;
+      const uint64_t x1 = spin[o1 + o] ^ qn, x2 = spin[o2 + o] ^ qn;
+      spin[o1 + o] = x1;
+      spin[o2 + o] = x2;
       qw[w] = x1;
       nw[w] = x2;
     }
@@ -1991,14 +2011,11 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   }
   __syncthreads();
   if (tid < 2) {
-    const int r = gv(in.pairs)[2 * grp + tid];
-    const size_t chain = (size_t)s * g.nr + r;
-    const int c = gv(a.wctb)[chain * g.nt + beta];
-    const int old = gv(a.ener)[chain * g.nt + c];
-    gv(a.ener)[chain * g.nt + c] = esum[tid];
+    const size_t ch = tid ? ch2 : ch1;
+    gv(a.ener)[ch * g.nt + (tid ? c2 : c1)] = esum[tid];
     // the sums already hold the pre-move energy; all values are integers
     // below 2^53, so adding the difference is exact
-    if (ph.measure) gv(a.esum)[chain * g.nt + beta] += (double)(esum[tid] - old);
+    if (ph.measure) gv(a.esum)[ch * g.nt + beta] = e_sum + (double)(esum[tid] - e_old);
   }
 }
 
