@@ -1504,12 +1504,19 @@ struct Rows {
 //     only where a run overlap starts (when the bond at x-1 joins the same
 //     two runs the union is implied).
 // z bonds leave the slab only when wrap_z (the whole periodic tube).
-// Row order of the linking loops: the lanes of a wave take rows nrows/64
-// apart (row rr of the loop is row (rr mod 64) * nrows/64 + rr / 64), so
-// that concurrent unions mostly meet different clusters instead of
-// contending for the same roots.  Identity when 64 does not divide nrows.
+// Row order of the linking loops: the lanes of a wave take rows about
+// nrows/64 apart, so that concurrent unions mostly meet different clusters
+// instead of contending for the same roots: lane l = rr mod 64 of loop row
+// rr takes row l * m + j of its range of m = nrows/64 rows, with
+// j = (rr / 64 + l / 2) mod m.  The rotation by l / 2 puts the lanes' rows
+// (and their run bases) in different LDS banks; without it rows m words
+// apart share a few banks.  Identity when 64 does not divide nrows.
 __device__ __forceinline__ int spread_row(int rr, int nrows) {
-  return (nrows & 63) ? rr : (rr & 63) * (nrows >> 6) + (rr >> 6);
+  if (nrows & 63) return rr;
+  const int m = nrows >> 6, lane = rr & 63;
+  int j = (rr >> 6) + (lane >> 1) % m;
+  if (j >= m) j -= m;
+  return lane * m + j;
 }
 
 template <typename I>
@@ -1726,26 +1733,27 @@ __device__ __forceinline__ void queue_links(const Lattice &g, const RowT *rows, 
         nz = node_base(rz);
       }
     }
-    for (;;) {
-      const bool has = (ty | tz) != 0;
-      const uint64_t m = __ballot(has);
-      if (!m) break;
-      if (has) {
-        const bool yb = ty != 0;
-        const int x = R::lo(yb ? ty : tz);
-        const uint32_t p0 = node(n0, b1, hbe, x);
-        const uint32_t p1 = yb ? node(ny, by1, hby, x) : node(nz, bz1, hbz, x);
-        if (yb)
-          ty &= ty - 1;
-        else
-          tz &= tz - 1;
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        wq[qn + (int)below] = p0 | p1 << 16;
+    // the y bonds of every lane's row, then the z bonds: one loop per
+    // direction needs a few more steps than one over both, but each step
+    // is shorter without choosing the direction's constants
+    auto bonds = [&](RowT t, uint32_t n2, RowT b21, int hbe2) {
+      for (;;) {
+        const bool has = t != 0;
+        const uint64_t m = __ballot(has);
+        if (!m) break;
+        if (has) {
+          const int x = R::lo(t);
+          t &= t - 1;
+          const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+          wq[qn + (int)below] = node(n0, b1, hbe, x) | node(n2, b21, hbe2, x) << 16;
+        }
+        qn += __popcll(m);
+        if (qn >= 64) drain(64);
       }
-      qn += __popcll(m);
-      if (qn >= 64) drain(64);
-    }
+    };
+    bonds(ty, ny, by1, hby);
+    bonds(tz, nz, bz1, hbz);
   }
   while (qn > 0) drain(qn < 64 ? qn : 64);
 }
