@@ -1807,8 +1807,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   uint32_t *rp = reinterpret_cast<uint32_t *>(base + lay.forest);    // run forest
   const Forest<IdxT, true> f{reinterpret_cast<IdxT *>(base + lay.forest)};  // site forest
   uint32_t *queue = reinterpret_cast<uint32_t *>(base + lay.region);  // while linking
-  uint64_t *nw = reinterpret_cast<uint64_t *>(base + lay.region);     // r2's new words
-  __shared__ int esum[2];
+  __shared__ int esum[1];
   __shared__ int wtot[16];
   __shared__ int nruns;
   const int beta = in.beta, s = in.s, grp = in.grp;
@@ -1823,13 +1822,13 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   const size_t ch1 = (size_t)s * g.nr + r1, ch2 = (size_t)s * g.nr + r2;
   const size_t sb = (size_t)s * 2 * g.half;
   const size_t o1 = sb + spin_row(g, r1, c1), o2 = sb + spin_row(g, r2, c2);
-  int e_old = 0;
+  int e_old1 = 0, e_old2 = 0;
   double e_sum = 0.0;
   if (tid < 2) {
-    const size_t ch = tid ? ch2 : ch1;
-    e_old = gv(a.ener)[ch * g.nt + (tid ? c2 : c1)];
-    if (ph.measure) e_sum = gv(a.esum)[ch * g.nt + beta];
-    esum[tid] = 0;
+    e_old1 = gv(a.ener)[ch1 * g.nt + c1];
+    e_old2 = gv(a.ener)[ch2 * g.nt + c2];
+    if (ph.measure) e_sum = gv(a.esum)[(tid ? ch2 : ch1) * g.nt + beta];
+    if (tid == 0) esum[0] = 0;
   }
   // the q words; the first of every thread is loaded (at a clamped index)
   // before the coins are drawn, so that its latency overlaps them
@@ -1982,48 +1981,47 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
     if (live && quarter == 0) {
       const size_t o = (size_t)p * g.half + h;
       const uint64_t qn = qw[w] ^ m;  // q' = q ^ coin(label)
-      const uint64_t x1 = spin[o1 + o] ^ qn, x2 = spin[o2 + o] ^ qn;
-      spin[o1 + o] = x1;
-      spin[o2 + o] = x2;
-      qw[w] = x1;
-      nw[w] = x2;
+      const uint64_t y1 = spin[o1 + o] ^ qn, y2 = spin[o2 + o] ^ qn;
+      spin[o1 + o] = y1;
+      spin[o2 + o] = y2;
+      qw[w] = y1;  // r1's new words, for its energy
     }
   }
   __syncthreads();
-  // energies of the two new configurations, from their even words
-  for (int t0 = 0; t0 < 2 * g.n; t0 += T) {  // every lane reaches the shuffles
-    const int t = t0 + tid;
-    int e0 = 0, e1 = 0;
-    if (t < 2 * g.n) {
-      const int rep = t >= g.n, h = t - rep * g.n;
-      const uint64_t *wd = rep ? nw : qw;
+  // The energy of r1's new configuration, from its even words.  r2's follows
+  // from E(r1) + E(r2), which the move conserves: every bond joins either
+  // two sites of one cluster (both flipped or neither) or two sites of
+  // unequal q, whose two replicas' terms cancel before and after (the
+  // open-boundary substitute bonds join y neighbours, bonds of the
+  // labelling too).
+  const int lane_e = tid & 63;
+  for (int t0 = 0; t0 < g.n; t0 += T) {
+    if (t0 + tid - lane_e >= g.n) break;  // the whole wave is past the words
+    const int h = t0 + tid;
+    int e0 = 0;
+    if (h < g.n) {
       const Nbrs nb = neighbours(g, 0, h);
       uint64_t sc[6];
-      load_neighbours(g, wd + g.n, 0, nb, 0, sc, 1);
-      const uint64_t sv = wd[h];
+      load_neighbours(g, qw + g.n, 0, nb, 0, sc, 1);
+      const uint64_t sv = qw[h];
       uint64_t Jw[6];
       load_couplings(g, a.J, s, 0, h, Jw);
       int u = 0;
 #pragma unroll
       for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
-      (rep ? e1 : e0) = 6 * 64 - 2 * u;
+      e0 = 6 * 64 - 2 * u;
     }
-    for (int off = 1; off < 64; off <<= 1) {
-      e0 += __shfl_xor(e0, off, 64);
-      e1 += __shfl_xor(e1, off, 64);
-    }
-    if ((tid & 63) == 0) {
-      if (e0) atomicAdd(esum, e0);
-      if (e1) atomicAdd(esum + 1, e1);
-    }
+    for (int off = 1; off < 64; off <<= 1) e0 += __shfl_xor(e0, off, 64);
+    if (lane_e == 0 && e0) atomicAdd(esum, e0);
   }
   __syncthreads();
   if (tid < 2) {
     const size_t ch = tid ? ch2 : ch1;
-    gv(a.ener)[ch * g.nt + (tid ? c2 : c1)] = esum[tid];
+    const int e1 = esum[0], e = tid ? e_old1 + e_old2 - e1 : e1;
+    gv(a.ener)[ch * g.nt + (tid ? c2 : c1)] = e;
     // the sums already hold the pre-move energy; all values are integers
     // below 2^53, so adding the difference is exact
-    if (ph.measure) gv(a.esum)[ch * g.nt + beta] = e_sum + (double)(esum[tid] - e_old);
+    if (ph.measure) gv(a.esum)[ch * g.nt + beta] = e_sum + (double)(e - (tid ? e_old2 : e_old1));
   }
 }
 

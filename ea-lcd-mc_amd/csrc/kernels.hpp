@@ -228,7 +228,7 @@ int hou_run_cap(const Lattice &g);
 // threads, run capacity cap, rows of rowb bytes and site-forest parents of
 // idxb bytes: q rows | q words | coin bits | first run of each row (u16) |
 // the run forest (u32) or the site forest | the waves' union queues, then
-// the runs' move bits, then replica r2's new words.
+// the runs' move bits (at most V/8 bytes, 16 n).
 struct HouWholeLds {
   uint32_t qw, cb, rbase, forest, region, total;
 };

@@ -160,6 +160,21 @@ def _ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
+_pairs_f = None
+
+
+def _pairs_fn():
+    """ea_houdayer_pairs taking plain addresses (no ctypes pointer objects
+    per call: the pairs of every step are drawn inside timed loops)."""
+    global _pairs_f
+    if _pairs_f is None:
+        f = lib()["ea_houdayer_pairs"]  # a function object of its own
+        f.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        f.restype = C.c_int
+        _pairs_f = f
+    return _pairs_f
+
+
 def check(rc):
     if rc != 0:
         raise EngineError("eatube error %d: %s" % (
@@ -512,9 +527,12 @@ class Simulation:
 
     def pairs(self, ncalls):
         out = np.zeros((ncalls, 4), np.int32)
-        for k in range(ncalls):
-            check(lib().ea_houdayer_pairs(C.byref(self.cl), self.nr,
-                                          _ptr(out[k], C.c_int32)))
+        f = _pairs_fn()
+        cl, base = C.addressof(self.cl), out.ctypes.data
+        for k in range(ncalls):  # plain addresses: one cheap call per draw
+            rc = f(cl, self.nr, base + 16 * k)
+            if rc:
+                check(rc)
         return out
 
     def schedule(self, n_it, ptfr=1, houfr=1, mesfr=1, cluster=True, it0=0,

@@ -108,6 +108,7 @@ def test_plain_small_config_with_pt():
     dict(L=8, Lz=16, nr=4, nt=6, ns=3, bc=1),
     dict(L=16, Lz=16, nr=4, nt=8, ns=2, bc=0),
     dict(L=8, Lz=24, nr=8, nt=4, ns=2, bc=1, variant=1),
+    dict(L=16, Lz=8, nr=4, nt=6, ns=2, bc=1, variant=2),
     dict(L=16, Lz=64, nr=4, nt=24, ns=2, bc=1, tmin=0.6, tmax=1.6),
 ])
 def test_steps_with_houdayer_match_oracle(case):
