@@ -576,7 +576,9 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_lane(const Tube *__restr
   uint64_t id1 = 0, id2 = 0;
   if (valid) {
     uint32_t w[6];
-#pragma unroll
+    // not unrolled: the words' Philox states would otherwise be live
+    // together and set the kernel's register count
+#pragma unroll 1
     for (int k = 0; k < 6; ++k) w[k] = sweep_word(g, ctr, lut, b.beta, h, r, k);
     thresholds(w, id1, id2);
   }
@@ -619,12 +621,15 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_lane(const Tube *__restr
 // widened by hs on each side (every x, y and unwrapped z neighbour), read
 // as 16-byte chunks from the even half site at or below its start: each
 // thread loads KC <= 2 chunks per sample instead of six neighbours.
-// Samples go in pairs; the next pair's tiles are loaded with this pair's
-// words and staged after the update (double buffer, barriers that leave
-// global traffic in flight).  All loads are unconditional (clamped
+// Samples go one at a time; the next sample's tile is loaded with this
+// sample's words and staged after the update (double buffer, barriers that
+// leave global traffic in flight).  All loads are unconditional (clamped
 // addresses), so they stay ahead of the one wait of the update instead of
 // sinking towards their use.  Only lanes whose z neighbour wraps around the
-// tube read it from global memory.
+// tube read it from global memory.  One sample in flight per thread (not
+// two) and the six random words made in a loop that is not unrolled keep
+// the kernel at 72 VGPRs, 7 waves per SIMD instead of 4: 230 -> 210 us per
+// half-sweep at L=48, Lz=384.
 constexpr int kTileWordsMax = 3;  // tile of at most 768 half sites
 
 __host__ __device__ inline int tile_width(const Lattice &g) { return kLaneThreads + 2 * g.hs; }
@@ -644,7 +649,7 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
   __shared__ __align__(16) uint32_t lut[EA_LUT_BYTES / 4];
   __shared__ uint32_t clone_off[EA_MAX_SAMPLES * EA_MAX_REPLICAS];
   __shared__ uint32_t ctr[EA_MAX_REPLICAS][5];
-  __shared__ __align__(16) uint64_t tile[2][2][2 * KC * kLaneThreads];  // [buffer][sample][x]
+  __shared__ __align__(16) uint64_t tile[2][2 * KC * kLaneThreads];  // [buffer][x]
   const Where wh = locate_block(tubes, map, ph.step,
                                 XCD ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x);
   const Tube &a = *wh.t;
@@ -691,94 +696,62 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
   const uint32_t jpair = 2 * (uint32_t)g.n;
 
   // row offsets are even (n is), so a chunk is 16-byte aligned
-  u64x2 tv[2][KC];
-  auto fetch = [&](int slot, int s) {
+  u64x2 tv[KC];
+  auto fetch = [&](int s) {
     const gword *S = gmem(spin + s * sstride) + clone_off[s * g.nr + r] + opp;
     const auto S2 = reinterpret_cast<const global_t<u64x2> *>(S);
 #pragma unroll
-    for (int i = 0; i < KC; ++i) tv[slot][i] = S2[th_[i] >> 1];
+    for (int i = 0; i < KC; ++i) tv[i] = S2[th_[i] >> 1];
   };
   auto stage = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < KC; ++i) {
-      *reinterpret_cast<u64x2 *>(&tile[buf][0][2 * tc_[i]]) = tv[0][i];
-      *reinterpret_cast<u64x2 *>(&tile[buf][1][2 * tc_[i]]) = tv[1][i];
-    }
+    for (int i = 0; i < KC; ++i) *reinterpret_cast<u64x2 *>(&tile[buf][2 * tc_[i]]) = tv[i];
   };
-
-  // the first pair's tiles are loaded before the random words are made,
-  // so that their latency overlaps that work
+  // the first sample's tile is loaded before the random words are made, so
+  // that its latency overlaps that work
   const int last = g.ns - 1;
-  fetch(0, 0);
-  fetch(1, min(1, last));
+  fetch(0);
   uint64_t id1 = 0, id2 = 0;
   if (valid) {
     uint32_t w[6];
-#pragma unroll
+    // not unrolled: the words' Philox states would otherwise be live
+    // together and set the kernel's register count
+#pragma unroll 1
     for (int k = 0; k < 6; ++k) w[k] = sweep_word(g, ctr, lut, b.beta, h, r, k);
     thresholds(w, id1, id2);
   }
   stage(0);
   lds_barrier();
-  for (int s0 = 0, buf = 0; s0 < g.ns; s0 += 2, buf ^= 1) {
-    const int s1 = min(s0 + 1, last);
-    const uint32_t c0 = clone_off[s0 * g.nr + r], c1 = clone_off[s1 * g.nr + r];
-    gword *S0 = gmem(spin + s0 * sstride), *S1 = gmem(spin + s1 * sstride);
-    fetch(0, min(s0 + 2, last));  // the next pair's tiles (after the last
-    fetch(1, min(s0 + 3, last));  // pair: repeats, not staged)
-    SiteIn x0, x1;
-    const gword *J0 = gmem(J + s0 * jstride), *J1 = gmem(J + s1 * jstride);
+  for (int s0 = 0, buf = 0; s0 < g.ns; ++s0, buf ^= 1) {
+    const uint32_t c0 = clone_off[s0 * g.nr + r];
+    gword *S0 = gmem(spin + s0 * sstride);
+    fetch(min(s0 + 1, last));  // the next sample's tile (after the last: a repeat)
+    SiteIn x0;
+    const gword *J0 = gmem(J + s0 * jstride);
     x0.sv = S0[c0 + ls.own];
-    x1.sv = S1[c1 + ls.own];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       x0.J[2 * k] = J0[ls.jo + k * jpair];
       x0.J[2 * k + 1] = J0[ls.jo + k * jpair + 1];
-      x1.J[2 * k] = J1[ls.jo + k * jpair];
-      x1.J[2 * k + 1] = J1[ls.jo + k * jpair + 1];
     }
-    uint64_t z0[2] = {0, 0}, z1[2] = {0, 0};
-    if (ti[2] < 0) {
-      z0[0] = S0[c0 + ls.nb[2]];
-      z1[0] = S1[c1 + ls.nb[2]];
-    }
-    if (ti[5] < 0) {
-      z0[1] = S0[c0 + ls.nb[5]];
-      z1[1] = S1[c1 + ls.nb[5]];
-    }
-    int e0 = 0, e1 = 0;
-    uint64_t n0 = 0, n1 = 0;
+    uint64_t z0[2] = {0, 0};
+    if (ti[2] < 0) z0[0] = S0[c0 + ls.nb[2]];
+    if (ti[5] < 0) z0[1] = S0[c0 + ls.nb[5]];
+    int e0 = 0;
+    uint64_t n0 = 0;
     if (valid) {
 #pragma unroll
-      for (int d = 0; d < 6; ++d) {
-        x0.sc[d] = ti[d] >= 0 ? tile[buf][0][ti[d]] : 0;
-        x1.sc[d] = ti[d] >= 0 ? tile[buf][1][ti[d]] : 0;
-      }
-      if (ti[2] < 0) {
-        x0.sc[2] = z0[0];
-        x1.sc[2] = z1[0];
-      }
-      if (ti[5] < 0) {
-        x0.sc[5] = z0[1];
-        x1.sc[5] = z1[1];
-      }
+      for (int d = 0; d < 6; ++d) x0.sc[d] = ti[d] >= 0 ? tile[buf][ti[d]] : 0;
+      if (ti[2] < 0) x0.sc[2] = z0[0];
+      if (ti[5] < 0) x0.sc[5] = z0[1];
       fix_lane(ls, p, x0);
-      fix_lane(ls, p, x1);
       n0 = x0.sv ^ site_flip(x0, id1, id2, ph.energy, e0);
-      n1 = x1.sv ^ site_flip(x1, id1, id2, ph.energy, e1);
     }
-    stage(buf ^ 1);  // after the last pair: into a buffer nobody reads
-    if (valid) {
-      S0[c0 + ls.own] = n0;
-      if (s1 != s0) S1[c1 + ls.own] = n1;
-    }
+    stage(buf ^ 1);  // after the last sample: into a buffer nobody reads
+    if (valid) S0[c0 + ls.own] = n0;
     if (ph.energy) {
       e0 = group_sum<64>(e0);
-      e1 = group_sum<64>(e1);
-      if (pwrite) {
-        part[(uint32_t)s0 * pstride + pbase] = e0;
-        if (s1 != s0) part[(uint32_t)s1 * pstride + pbase] = e1;
-      }
+      if (pwrite) part[(uint32_t)s0 * pstride + pbase] = e0;
     }
     lds_barrier();
   }
