@@ -16,7 +16,10 @@ the chunks of a run are its consecutive steps.
 The reference has no counterpart (it runs one tube size per process,
 PROGRAM/Makefile:3-36); each request is that program's run of one job.
 """
+import argparse
+import json
 import math
+import sys
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -139,12 +142,14 @@ class Server:
                arguments); the default suits requests of one sample: one
                thread per position makes its random words and updates the
                sample (sweep kind 1)
+    progress   called as progress(seconds, finished, running, pending) at
+               most every progress_every seconds, between chunks
     """
 
     def __init__(self, budget, chunk=32, max_tubes=256, keep=False,
                  start_fn=start, batch_fn=eatube.Batch, footprint_fn=footprint,
                  clock=time.perf_counter, workers=0, prepare_fn=prepare,
-                 shape=(64, 512, 1, 0)):
+                 shape=(64, 512, 1, 0), progress=None, progress_every=30.0):
         self.budget = int(budget)
         self.chunk = int(chunk)
         self.max_tubes = int(max_tubes)
@@ -153,6 +158,7 @@ class Server:
         self.footprint_fn, self.clock = footprint_fn, clock
         self.workers, self.prepare_fn = int(workers), prepare_fn
         self.shape = tuple(shape) if shape else None
+        self.progress, self.progress_every = progress, float(progress_every)
         self.luts = _Luts()
         self.stats = dict(chunks=0, batches_built=0, max_running=0,
                           max_bytes=0, deferred=0, t_start=0.0, t_batch=0.0,
@@ -191,9 +197,13 @@ class Server:
         used = 0
         results = {}
         batch, members = None, ()
+        told = 0.0
         try:
             while pending or running:
                 now = self.clock() - t0
+                if self.progress is not None and now - told >= self.progress_every:
+                    told = now
+                    self.progress(now, len(results), len(running), len(pending))
                 if pool is not None:  # arrived requests start preparing at once
                     for r in pending:
                         if r.arrival > now:
@@ -293,3 +303,85 @@ def summary(requests, results, wall):
             "mean_latency_s": float(lat.mean()),
             "wall_s": wall,
             "attempts_per_ns": work / (wall * 1e9)}
+
+
+def record(r, x):
+    """One finished request as a JSON-able record."""
+    return {"rid": r.rid, "L": r.L, "Lz": r.Lz, "bc": r.bc, "nt": r.nt,
+            "sweeps": r.sweeps, "attempts": r.attempts(), "arrival": x.arrival,
+            "admitted": x.admitted, "finished": x.finished, "latency": x.latency}
+
+
+def merge(paths):
+    """The summary of several runs' records files (segments of one stream):
+    latency percentiles over all requests, throughput = all attempts over
+    the sum of the segments' walls."""
+    recs, wall = [], 0.0
+    for p in paths:
+        with open(p) as f:
+            d = json.load(f)
+        recs += d["records"]
+        wall += d["wall_s"]
+    lat = np.array([x["latency"] for x in recs])
+    return {"requests": len(recs), "segments": len(paths),
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()), "wall_s": wall,
+            "attempts": float(sum(x["attempts"] for x in recs)),
+            "attempts_per_ns": sum(x["attempts"] for x in recs) / (wall * 1e9)}
+
+
+def main(argv=None):
+    """Serve a Poisson stream of requests (or a segment of one: requests
+    first .. first+count-1 of the same seeded stream, their arrival times
+    taken from the previous request's), print progress and a JSON summary,
+    and write every request's record to --out; --merge prints the summary
+    of several such files."""
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=0.5, help="arrivals per second")
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--first", type=int, default=0)
+    ap.add_argument("--count", type=int, default=0, help="0: to the end")
+    ap.add_argument("--sweeps", default="1000,100000", help="log-uniform range")
+    ap.add_argument("--chunk", type=int, default=128)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--out", default="")
+    ap.add_argument("--merge", nargs="*", default=None)
+    args = ap.parse_args(argv)
+    if args.merge is not None:
+        print(json.dumps(merge(args.merge)))
+        return 0
+    lo, hi = (int(x) for x in args.sweeps.split(","))
+    reqs = poisson_requests(args.requests, args.rate, seed=args.seed, sweeps=(lo, hi))
+    last = args.requests if args.count <= 0 else min(args.requests, args.first + args.count)
+    seg = reqs[args.first:last]
+    t_prev = reqs[args.first - 1].arrival if args.first > 0 else 0.0
+    for r in seg:
+        r.arrival -= t_prev
+
+    def say(t, done, running, pending):
+        print("%.0f s: %d finished, %d running, %d waiting" % (t, done, running, pending),
+              flush=True)
+
+    srv = Server(budget=int(args.budget_gb * 2 ** 30), chunk=args.chunk,
+                 workers=args.workers, progress=say, progress_every=60.0)
+    t0 = time.perf_counter()
+    res = srv.run(seg)
+    wall = time.perf_counter() - t0
+    out = summary(seg, res, wall)
+    out.update({"first": args.first, "count": len(seg), "rate_per_s": args.rate,
+                "seed": args.seed, "sweeps_range": [lo, hi], "chunk_steps": args.chunk,
+                "max_running": srv.stats["max_running"],
+                "batches_built": srv.stats["batches_built"]})
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump({"summary": out, "wall_s": wall,
+                       "records": [record(r, x) for r, x in zip(seg, res)]}, f)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
