@@ -1906,18 +1906,29 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
       }
     }
     __syncthreads();
-    // the move mask of each row: every site takes its run's move bit
+    // the move mask of each row: every site takes its run's move bit.  The
+    // mask toggles where a run starts whose bit differs from the run before
+    // (and at the start of the run that wraps, which is run 0 again); a
+    // prefix XOR of the toggles then fills the runs
     for (int r = tid; r < nrows; r += T) {
-      const RowT b = R::bounds(rows[r], L, full), b1 = b | 1;
+      const RowT b = R::bounds(rows[r], L, full);
       const int hbe = R::wrap_start(b, L);
       const uint32_t id0 = rbase[r], sh = id0 & 63;
       const int nr_ = R::runs(b);
       const uint64_t w0 = mv[id0 >> 6];
       const uint64_t w1 = sh + nr_ > 64 ? mv[(id0 >> 6) + 1] : 0;
       const uint64_t rm = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;  // bit k: run k moves
-      RowT m = 0;
-      for (int x = 0; x < L; ++x) m |= (RowT)((rm >> R::rank(b1, hbe, x)) & 1u) << x;
-      rows[r] = m;  // only this thread reads row r in this loop
+      const uint64_t dm = rm ^ (rm >> 1);  // bit k: runs k and k+1 differ
+      RowT st = b & ~RowT(1);  // the starts of runs 1 .. nr_-1, in order
+      if (hbe < L) st &= ~(RowT(1) << hbe);
+      RowT t = hbe < L ? (RowT)(((rm >> (nr_ - 1)) ^ rm) & 1u) << hbe : RowT(0);
+      for (int k = 0; st; ++k) {
+        t |= (RowT)((dm >> k) & 1u) << R::lo(st);
+        st &= st - 1;
+      }
+#pragma unroll
+      for (int sft = 1; sft < (int)(8 * sizeof(RowT)); sft <<= 1) t ^= t << sft;
+      rows[r] = (t ^ ((rm & 1u) ? full : RowT(0))) & full;  // only this thread reads row r here
     }
   } else {
     for (int i = tid; i < V; i += T) f.p[i] = f.ix(i);  // every site its own root
