@@ -161,10 +161,9 @@ def batched_compare(steps, samples=1, ntubes=48):
         p.engine.sync()
     scheds = [p.schedule(steps, cluster=True) for p in plain]
     t0 = time.perf_counter()
-    for p, sch in zip(plain, scheds):
-        p.engine.run(sch)
-    for p in plain:
-        p.engine.sync()
+    for p, sch in zip(plain, scheds):  # each tube's run completes before the next
+        p.engine.run(sch)               # starts (ea_run returns once enqueued, and
+        p.engine.sync()                 # engines have streams of their own)
     t_plain = time.perf_counter() - t0
     # batched form, in the launch shape the autotuner finds fastest for
     # this mix (timed on scratch tubes of the same configurations)
