@@ -1483,10 +1483,12 @@ struct Rows {
 // rr takes row l * m + j of its range of m = nrows/64 rows, with
 // j = (rr / 64 + l / 2) mod m.  The rotation by l / 2 puts the lanes' rows
 // (and their run bases) in different LDS banks; without it rows m words
-// apart share a few banks.  Identity when 64 does not divide nrows.
+// apart share a few banks.  Rows from 64 * m on (m = nrows / 64 rounded
+// down) keep their place.
 __device__ __forceinline__ int spread_row(int rr, int nrows) {
-  if (nrows & 63) return rr;
-  const int m = nrows >> 6, lane = rr & 63;
+  const int m = nrows >> 6;
+  if (rr >= m << 6) return rr;
+  const int lane = rr & 63;
   int j = (rr >> 6) + (lane >> 1) % m;
   if (j >= m) j -= m;
   return lane * m + j;
@@ -2024,7 +2026,8 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
 //     slab's 16-bit pieces of the two replicas' spin words (piece bz of a
 //     word holds its sites in z block bz, and each slab owns the pieces of
 //     its planes, so blocks never write the same bits).
-//  4. k_hou_energy recomputes the two changed configurations' energies.
+//  4. k_hou_energy recomputes r1's changed energy; r2's follows from the
+//     conserved sum E(r1) + E(r2).
 __device__ __forceinline__ size_t instance_index(const Lattice &g, const Inst &in) {
   return ((size_t)in.grp * g.ns + in.s) * g.ctm + in.beta;
 }
@@ -2334,7 +2337,7 @@ __global__ void __launch_bounds__(256) k_hou_apply(const Tube *__restrict__ tube
 // measures, their energy sums move by the difference.
 __global__ void __launch_bounds__(512) k_hou_energy(const Tube *__restrict__ tubes,
                                                     const uint64_t *__restrict__ map, Phase ph) {
-  __shared__ int esum[2];
+  __shared__ int esum[1];
   const Where wh = locate(tubes, map, ph.step);
   const Tube &a = *wh.t;
   if (wh.step >= a.nsteps) return;
@@ -2342,12 +2345,23 @@ __global__ void __launch_bounds__(512) k_hou_energy(const Tube *__restrict__ tub
   const Inst in = instance_of(a, wh.local, wh.step, ph.call);
   const Pair pw = pair_words(g, a, in.pairs, in.s, in.grp, in.beta);
   const int T = blockDim.x, tid = threadIdx.x;
-  if (tid < 2) esum[tid] = 0;
+  // the two chains' energies before the move, read here so that the end only
+  // stores
+  const int r1 = gv(in.pairs)[2 * in.grp], r2 = gv(in.pairs)[2 * in.grp + 1];
+  const size_t ch1 = (size_t)in.s * g.nr + r1, ch2 = (size_t)in.s * g.nr + r2;
+  const int c1 = gv(a.wctb)[ch1 * g.nt + in.beta], c2 = gv(a.wctb)[ch2 * g.nt + in.beta];
+  int e_old1 = 0, e_old2 = 0;
+  if (tid < 2) {
+    e_old1 = gv(a.ener)[ch1 * g.nt + c1];
+    e_old2 = gv(a.ener)[ch2 * g.nt + c2];
+  }
+  if (tid == 0) esum[0] = 0;
   __syncthreads();
-  int e0 = 0, e1 = 0;
-  for (int t = tid; t < 2 * g.n; t += T) {
-    const int rep = t >= g.n, h = t - rep * g.n;
-    const auto base = gmem(a.spin) + (rep ? pw.o2 : pw.o1);
+  // r1's new energy from its even words; r2's follows from the sum, which
+  // the move conserves (as in k_hou_whole)
+  int e0 = 0;
+  for (int h = tid; h < g.n; h += T) {
+    const auto base = gmem(a.spin) + pw.o1;
     const Nbrs nb = neighbours(g, 0, h);
     uint64_t sc[6], Jw[6];
     load_neighbours(g, base + g.half, 0, nb, 0, sc, 1);
@@ -2356,24 +2370,18 @@ __global__ void __launch_bounds__(512) k_hou_energy(const Tube *__restrict__ tub
     int u = 0;
 #pragma unroll
     for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
-    (rep ? e1 : e0) += 6 * 64 - 2 * u;
+    e0 += 6 * 64 - 2 * u;
   }
   e0 = wave_sum(e0);
-  e1 = wave_sum(e1);
-  if ((tid & 63) == 0) {
-    if (e0) atomicAdd(esum, e0);
-    if (e1) atomicAdd(esum + 1, e1);
-  }
+  if ((tid & 63) == 0 && e0) atomicAdd(esum, e0);
   __syncthreads();
   if (tid < 2) {
-    const int r = gv(in.pairs)[2 * in.grp + tid];
-    const size_t chain = (size_t)in.s * g.nr + r;
-    const int c = gv(a.wctb)[chain * g.nt + in.beta];
-    const int old = gv(a.ener)[chain * g.nt + c];
-    gv(a.ener)[chain * g.nt + c] = esum[tid];
+    const size_t chain = tid ? ch2 : ch1;
+    const int e1 = esum[0], e = tid ? e_old1 + e_old2 - e1 : e1;
+    gv(a.ener)[chain * g.nt + (tid ? c2 : c1)] = e;
     // the sums already hold the pre-move energy; all values are integers
     // below 2^53, so adding the difference is exact
-    if (ph.measure) gv(a.esum)[chain * g.nt + in.beta] += (double)(esum[tid] - old);
+    if (ph.measure) gv(a.esum)[chain * g.nt + in.beta] += (double)(e - (tid ? e_old2 : e_old1));
   }
 }
 
