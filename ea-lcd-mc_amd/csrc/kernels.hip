@@ -1627,19 +1627,20 @@ __device__ __forceinline__ uint32_t run_root(uint32_t *p, uint32_t a) {
 // and the walk goes on from the node's old parent.  It ends when both nodes
 // have one parent, or at a link; concurrent unions only lower parents
 // further, so every node stays connected to what it was joined with.
-__device__ __forceinline__ void rem_unite(uint32_t *p, uint32_t a, uint32_t b) {
-  for (;;) {
-    const uint32_t ea = run_entry(p, a), eb = run_entry(p, b);
-    if ((ea ^ eb) < 0x10000u) return;  // one parent
-    // x: the node with the larger parent, y the other (selects, no branch)
-    const bool sw = ea < eb;
-    const uint32_t x = sw ? b : a, ex = sw ? eb : ea, ey = sw ? ea : eb;
-    b = sw ? a : b;
-    const uint32_t old = __hip_atomic_fetch_min(p + x, (ey & 0xFFFF0000u) | (ex & 0xFFFFu),
-                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if ((old >> 16) == x) return;  // x was a root: linked
-    a = old >> 16;
-  }
+// One step of the walk; true when the union is complete.  The pair (a, b)
+// is the walk's whole state, so a union left unfinished can be resumed
+// later from the pair, after any number of other unions.
+__device__ __forceinline__ bool rem_step(uint32_t *p, uint32_t &a, uint32_t &b) {
+  const uint32_t ea = run_entry(p, a), eb = run_entry(p, b);
+  if ((ea ^ eb) < 0x10000u) return true;  // one parent
+  // x: the node with the larger parent, y the other (selects, no branch)
+  const bool sw = ea < eb;
+  const uint32_t x = sw ? b : a, ex = sw ? eb : ea, ey = sw ? ea : eb;
+  b = sw ? a : b;
+  const uint32_t old = __hip_atomic_fetch_min(p + x, (ey & 0xFFFF0000u) | (ex & 0xFFFFu),
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  a = old >> 16;
+  return a == x;  // x was a root: linked
 }
 
 // The unions of a whole tube's labelling: for every row r, the y and z bonds
@@ -1647,12 +1648,15 @@ __device__ __forceinline__ void rem_unite(uint32_t *p, uint32_t a, uint32_t b) {
 // the same two runs the union is implied), between the nodes of the runs
 // they join.  A lane finds the bonds of its rows and pushes the node pairs
 // onto its wave's queue (kHouQueue entries of two 16-bit nodes); once 64
-// are queued the wave unites them, one per lane, so every lane stays busy
-// however unevenly the bonds fall on the rows.  BY_RUNS: nodes are run
-// numbers (the first run number of row r, rbase[r], plus the run's rank);
-// otherwise sites (the run's first site, Rows::rep_at).  Lanes take rows
-// nrows/64 apart (spread_row), so concurrent unions mostly meet different
-// clusters.
+// are queued the wave takes them, one per lane, so every lane stays busy
+// however unevenly the bonds fall on the rows.  unite(u, v) works on the
+// pair and returns whether the union is complete; a pair it leaves
+// unfinished (its walk state, which it updates) goes back onto the queue,
+// so that a drain takes a fixed number of steps instead of as many as its
+// slowest union needs.  BY_RUNS: nodes are run numbers (the first run
+// number of row r, rbase[r], plus the run's rank); otherwise sites (the
+// run's first site, Rows::rep_at).  Lanes take rows nrows/64 apart
+// (spread_row), so concurrent unions mostly meet different clusters.
 template <bool BY_RUNS, typename RowT, typename Unite>
 __device__ __forceinline__ void queue_links(const Lattice &g, const RowT *rows, RowT full,
                                             const uint16_t *rbase, uint32_t *queue,
@@ -1661,14 +1665,28 @@ __device__ __forceinline__ void queue_links(const Lattice &g, const RowT *rows, 
   const int L = g.L, nrows = g.V / L, T = blockDim.x, tid = threadIdx.x, lane = tid & 63;
   uint32_t *wq = queue + (tid >> 6) * kHouQueue;
   int qn = 0;  // pairs in the wave's queue, the same in every lane
+  // takes the top cnt <= 64 pairs; with qn < kHouQueue before, qn stays
+  // below kHouQueue (at most 64 come back)
   auto drain = [&](int cnt) {
     lds_wave_sync();  // the queue's writes are done
-    if (lane < cnt) {
-      const uint32_t v = wq[qn - cnt + lane];
-      unite(v & 0xFFFFu, v >> 16);
+    uint32_t u = 0, v = 0;
+    const bool got = lane < cnt;
+    if (got) {
+      const uint32_t e = wq[qn - cnt + lane];
+      u = e & 0xFFFFu;
+      v = e >> 16;
     }
     qn -= cnt;
     lds_wave_sync();  // and read before the slots are written again
+    const bool left = got && !unite(u, v);
+    const uint64_t m = __ballot(left);
+    if (m) {  // the unfinished pairs, back onto the queue
+      if (left)
+        wq[qn + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+            u | v << 16;
+      qn += __popcll(m);
+    }
   };
   auto node_base = [&](int r) { return BY_RUNS ? (uint32_t)rbase[r] : (uint32_t)(r * L); };
   auto node = [&](uint32_t nb, RowT b1, int hbe, int x) {
@@ -1710,21 +1728,34 @@ __device__ __forceinline__ void queue_links(const Lattice &g, const RowT *rows, 
     }
     // the y bonds of every lane's row, then the z bonds: one loop per
     // direction needs a few more steps than one over both, but each step
-    // is shorter without choosing the direction's constants
+    // is shorter without choosing the direction's constants.  A lane
+    // pushes up to two bonds per step (its two lowest), so the loop takes
+    // half as many steps as the lane with the most bonds has; a step starts
+    // with fewer than 64 pairs queued and adds at most 128.
     auto bonds = [&](RowT t, uint32_t n2, RowT b21, int hbe2) {
+      auto below = [](uint64_t m) {
+        return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      };
       for (;;) {
-        const bool has = t != 0;
-        const uint64_t m = __ballot(has);
-        if (!m) break;
-        if (has) {
+        const bool has1 = t != 0;
+        const uint64_t m1 = __ballot(has1);
+        if (!m1) break;
+        const RowT t2 = t & (t - 1);
+        const bool has2 = t2 != 0;
+        const uint64_t m2 = __ballot(has2);
+        const int slot = qn + below(m1) + below(m2);
+        if (has1) {
           const int x = R::lo(t);
-          t &= t - 1;
-          const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-              (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-          wq[qn + (int)below] = node(n0, b1, hbe, x) | node(n2, b21, hbe2, x) << 16;
+          wq[slot] = node(n0, b1, hbe, x) | node(n2, b21, hbe2, x) << 16;
         }
-        qn += __popcll(m);
-        if (qn >= 64) drain(64);
+        if (has2) {
+          const int x = R::lo(t2);
+          wq[slot + 1] = node(n0, b1, hbe, x) | node(n2, b21, hbe2, x) << 16;
+        }
+        t = t2 & (t2 - 1);
+        qn += __popcll(m1) + __popcll(m2);
+        while (qn >= 64) drain(64);
       }
     };
     bonds(ty, ny, by1, hby);
@@ -1878,8 +1909,11 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   const bool by_runs = nrun <= cap;
 
   if (by_runs) {
-    queue_links<true>(g, rows, full, rbase, queue,
-                      [&](uint32_t u, uint32_t v) { rem_unite(rp, u, v); });
+    // two steps of Rem's walk per drain: most unions end within them (1.5
+    // to 1.8 steps on average on uncorrelated q), the rest resume later
+    queue_links<true>(g, rows, full, rbase, queue, [&](uint32_t &u, uint32_t &v) {
+      return rem_step(rp, u, v) || rem_step(rp, u, v);
+    });
     __syncthreads();
     if (a.keep) {
       const auto out = gv(a.keep) + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
@@ -1935,8 +1969,10 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   } else {
     for (int i = tid; i < V; i += T) f.p[i] = f.ix(i);  // every site its own root
     __syncthreads();
-    queue_links<false>(g, rows, full, rbase, queue,
-                       [&](uint32_t u, uint32_t v) { f.unite(u, v); });
+    queue_links<false>(g, rows, full, rbase, queue, [&](uint32_t &u, uint32_t &v) {
+      f.unite(u, v);
+      return true;
+    });
     __syncthreads();
     if (a.keep) {
       const auto out = gv(a.keep) + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;

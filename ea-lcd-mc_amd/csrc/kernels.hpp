@@ -219,7 +219,7 @@ hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
 // LDS bytes of the whole-tube and slab Houdayer kernels; a tube is labelled
 // in one workgroup or in slabs as hou_use_slabs decides.  Both need
 // L <= 64.
-constexpr int kHouQueue = 128;  // union queue entries per wave (whole-tube kernel)
+constexpr int kHouQueue = 192;  // union queue entries per wave (whole-tube kernel)
 // Run nodes the whole-tube labelling holds (Tube::hou_cap): every run of a
 // small tube, about 9/16 of the sites otherwise (a tube of uncorrelated q
 // has about V/2 runs); an instance with more runs is labelled over sites.
