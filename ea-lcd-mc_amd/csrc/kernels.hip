@@ -1556,19 +1556,21 @@ __device__ void link_rows(const Lattice &g, I *lab, const uint64_t *rows, int pl
 }
 
 // q bits of row r = y + L z from the two replicas' words: the L sites of a
-// row use ML words (parity alternating with x), each for its 4 x copies (bx).
-template <typename Load>
-__device__ __forceinline__ uint64_t row_bits(const Lattice &g, int r, Load load) {
+// row use ML words (parity alternating with x), each for its 4 x copies (bx),
+// which are the 4 consecutive bits 4 by + 16 bz ... +3 of the word.  RowT is
+// the row's width (32 bits when L <= 32).
+template <typename RowT = uint64_t, typename Load>
+__device__ __forceinline__ RowT row_bits(const Lattice &g, int r, Load load) {
   const int z = qdiv(r, g.mL), y = r - z * g.L;
   const int by = qdiv(y, g.mML), yb = y - by * g.ML;
   const int bz = qdiv(z, g.mMLz), zg = z - bz * g.MLz;
   const int hrow = g.ML * yb + g.MS * zg, bitrow = 4 * by + 16 * bz;
   const int prow = (y ^ z) & 1;
-  uint64_t bits = 0;
+  RowT bits = 0;
   for (int ax = 0; ax < g.ML; ++ax) {
-    const uint64_t q = load((ax + prow) & 1, (ax + hrow) >> 1) >> bitrow;
+    const uint32_t nib = (uint32_t)(load((ax + prow) & 1, (ax + hrow) >> 1) >> bitrow);
 #pragma unroll
-    for (int bx = 0; bx < 4; ++bx) bits |= ((q >> bx) & 1ULL) << (bx * g.ML + ax);
+    for (int bx = 0; bx < 4; ++bx) bits |= (RowT)((nib >> bx) & 1u) << (bx * g.ML + ax);
   }
   return bits;
 }
@@ -1867,7 +1869,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
     const int per = (nrows + T - 1) / T, r0 = min(tid * per, nrows), r1 = min(r0 + per, nrows);
     int own = 0;
     for (int r = r0; r < r1; ++r) {
-      const RowT q = (RowT)row_bits(g, r, [&](int p, int h) { return qw[p * g.n + h]; });
+      const RowT q = row_bits<RowT>(g, r, [&](int p, int h) { return qw[p * g.n + h]; });
       rows[r] = q;
       own += R::runs(R::bounds(q, L, full));
     }
@@ -1908,6 +1910,21 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   const int nrun = nruns;
   const bool by_runs = nrun <= cap;
 
+  // The apply's first spin words (of thread tid's word, quarter 0) and the
+  // energy's first couplings (half site tid), loaded once the links are made
+  // so that their latency overlaps the move bits.  No other workgroup of the
+  // launch writes the two replicas' words at this beta, and this one writes
+  // them only in the apply.
+  uint64_t pre1 = 0, pre2 = 0, Jpre[6] = {0, 0, 0, 0, 0, 0};
+  auto prefetch = [&]() {
+    if (tid < 8 * g.n && (tid & 3) == 0) {
+      const int w = tid >> 2, p = w >= g.n, h = w - p * g.n;
+      const size_t o = (size_t)p * g.half + h;
+      pre1 = spin[o1 + o];
+      pre2 = spin[o2 + o];
+    }
+    if (tid < g.n) load_couplings(g, a.J, s, 0, tid, Jpre);
+  };
   if (by_runs) {
     // two steps of Rem's walk per drain: most unions end within them (1.5
     // to 1.8 steps on average on uncorrelated q), the rest resume later
@@ -1915,6 +1932,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
       return rem_step(rp, u, v) || rem_step(rp, u, v);
     });
     __syncthreads();
+    prefetch();
     if (a.keep) {
       const auto out = gv(a.keep) + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
       for (int i = tid; i < V; i += T) {
@@ -1974,6 +1992,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
       return true;
     });
     __syncthreads();
+    prefetch();
     if (a.keep) {
       const auto out = gv(a.keep) + (((size_t)grp * g.ns + s) * g.ctm + beta) * V;
       for (int i = tid; i < V; i += T) {
@@ -2003,7 +2022,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
     if (live && quarter == 0) {
       const size_t o = (size_t)p * g.half + h;
       const uint64_t qn = qw[w] ^ m;  // q' = q ^ coin(label)
-      const uint64_t y1 = spin[o1 + o] ^ qn, y2 = spin[o2 + o] ^ qn;
+      const uint64_t y1 = (t0 ? spin[o1 + o] : pre1) ^ qn, y2 = (t0 ? spin[o2 + o] : pre2) ^ qn;
       spin[o1 + o] = y1;
       spin[o2 + o] = y2;
       qw[w] = y1;  // r1's new words, for its energy
@@ -2027,7 +2046,12 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
       load_neighbours(g, qw + g.n, 0, nb, 0, sc, 1);
       const uint64_t sv = qw[h];
       uint64_t Jw[6];
-      load_couplings(g, a.J, s, 0, h, Jw);
+      if (t0) {
+        load_couplings(g, a.J, s, 0, h, Jw);
+      } else {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) Jw[d] = Jpre[d];
+      }
       int u = 0;
 #pragma unroll
       for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
