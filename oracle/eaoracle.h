@@ -32,6 +32,12 @@
  *   orc_energy_msc ............ pinned: == reference scalar energy (med.c)
  *   orc_pt .................... pinned: fixtures from upd.c built in-container
  *   orc_labels ................ pinned: BFS minimum-index brute force
+ *   orc_sim_houdayer's replica
+ *   pairs ..................... pinned: the cluster stream by fixtures from
+ *                                the reference host C; NR/2-1 draws per group
+ *                                per call, the pair as the first two entries
+ *                                of a forward Fisher-Yates shuffle in closed
+ *                                form, and uniform ordered pairs (chi-square)
  *   orc_coin .................. substitute generator (reference uses closed
  *                                cuRAND): parity unpinned vs the reference
  *   orc_make_lut .............. pinned: byte-identical to the shipped

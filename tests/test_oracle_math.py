@@ -14,6 +14,13 @@ known answers, brute force and closed forms (not by re-typing them).
 - Random words (main.cu:1095-1150): the bits of each byte must be Bernoulli
   with p = exp(-4 beta) (create_LUT.c:249-263) and the three words
   independent.  The post-Philox mix and rotation are otherwise unpinned.
+- Replica pairs of the Houdayer move (main.cu:409-421): the cluster stream
+  itself is pinned by the reference's fixtures; here each call must take
+  exactly NR/2 - 1 draws per group, the pair must be the first two entries
+  of a forward Fisher-Yates shuffle of the group driven by the draws' high
+  words (a closed form: entry 0 is position j0, entry 1 is position j1
+  unless j1 = j0, where it is position 0), and over many calls every ordered
+  pair of distinct replicas of a group must be equally likely.
 """
 import ctypes as C
 
@@ -295,3 +302,56 @@ def test_it_loop_counts(ptfr, houfr, mesfr):
     want = np.take_along_axis(e, w.astype(np.int64), axis=2)
     assert np.array_equal(o2.field("esum").reshape(ns, nr, nt), want)
     assert np.array_equal(h2[0], o2.field("wbtc").reshape(ns, nr, nt))
+
+
+def _tiny_cluster_sim(nr, seed_cl):
+    betas = orc.betas_linear(0.6, 1.6, 2)
+    return orc.Sim(8, 8, nr, 2, 1, seeds=(5, 6, 7, seed_cl), betas=betas)
+
+
+@pytest.mark.parametrize("nr", [4, 8, 16])
+def test_houdayer_pairs_are_fisher_yates_heads(nr):
+    """The oracle's pair choice against the cluster stream: draws per call,
+    and the first two entries of the group's shuffle in closed form."""
+    h = nr // 2
+    s = _tiny_cluster_sim(nr, 4242)
+    cl = orc.HQ.from_buffer_copy(bytes(s.field("cl")))
+    for call in range(12):
+        s.houdayer()
+        got = s.field("last_pairs")
+        for grp in range(2):
+            u = [int(orc.lib().orc_hq_next(C.byref(cl))) >> 32 for _ in range(h - 1)]
+            j0 = u[0] % h
+            if h == 2:
+                first, second = j0, 1 - j0
+            else:
+                j1 = 1 + u[1] % (h - 1)
+                first, second = j0, (0 if j1 == j0 else j1)
+            assert (got[2 * grp], got[2 * grp + 1]) == (grp * h + first, grp * h + second), \
+                (nr, call, grp)
+        # the call took exactly 2 (NR/2 - 1) draws of the cluster stream
+        assert bytes(s.field("cl")) == orc.hq_to_bytes(cl), (nr, call)
+
+
+@pytest.mark.parametrize("nr", [4, 8])
+def test_houdayer_pairs_are_uniform(nr):
+    """Every ordered pair of distinct replicas of a group is equally likely
+    (chi-square at the 0.1% level over 4000 calls)."""
+    from scipy.stats import chi2
+    h = nr // 2
+    s = _tiny_cluster_sim(nr, 777)
+    counts = np.zeros((2, h, h), np.int64)
+    ncalls = 4000
+    for _ in range(ncalls):
+        s.houdayer()
+        got = s.field("last_pairs")
+        for grp in range(2):
+            r1, r2 = int(got[2 * grp]) - grp * h, int(got[2 * grp + 1]) - grp * h
+            assert 0 <= r1 < h and 0 <= r2 < h and r1 != r2
+            counts[grp, r1, r2] += 1
+    cells = h * (h - 1)
+    expect = ncalls / cells
+    for grp in range(2):
+        obs = counts[grp][~np.eye(h, dtype=bool)]
+        stat = float(((obs - expect) ** 2 / expect).sum())
+        assert stat < chi2.ppf(0.999, cells - 1), (grp, stat)
