@@ -160,6 +160,25 @@ def _ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
+# Counter blocks as numpy records (the layout of Counter), so that a call's
+# blocks need no new ctypes array type per count.
+COUNTER_DTYPE = np.dtype([("x", np.uint32), ("y", np.uint32), ("z", np.uint32),
+                          ("w", np.uint32)])
+
+
+def _counters_arg(blocks):
+    """(pointer, count) of counter blocks given as a COUNTER_DTYPE array, a
+    ctypes Counter array or a sequence of Counters (the pointer keeps the
+    array it points into alive)."""
+    if isinstance(blocks, np.ndarray):
+        if blocks.dtype != COUNTER_DTYPE or not blocks.flags.c_contiguous:
+            raise ValueError("counter blocks must be a contiguous COUNTER_DTYPE array")
+        return _ptr(blocks, Counter), len(blocks)
+    if not isinstance(blocks, C.Array):
+        blocks = (Counter * len(blocks))(*blocks)
+    return C.cast(blocks, C.POINTER(Counter)), len(blocks)
+
+
 _pairs_f = None
 
 
@@ -337,17 +356,13 @@ class Engine:
         return st, acc, att, raw
 
     def sweep(self, blocks):
-        b = (Counter * len(blocks))(*blocks) if not isinstance(
-            blocks, C.Array) else blocks
-        nsw = len(b) // (2 * self.nr)
-        check(lib().ea_sweep(self.h, nsw, b))
+        b, n = _counters_arg(blocks)
+        check(lib().ea_sweep(self.h, n // (2 * self.nr), b))
 
     def sweep_noalloc(self, blocks):
         """ea_sweep through the per-GPU counter buffer (no allocation)."""
-        b = (Counter * len(blocks))(*blocks) if not isinstance(
-            blocks, C.Array) else blocks
-        nsw = len(b) // (2 * self.nr)
-        check(lib().ea_sweep_noalloc(self.h, nsw, b))
+        b, n = _counters_arg(blocks)
+        check(lib().ea_sweep_noalloc(self.h, n // (2 * self.nr), b))
 
     def energy(self):
         out = np.zeros((self.ns, self.nr, self.nt), np.int32)
@@ -519,9 +534,9 @@ class Simulation:
         self.rng_offset = 0
 
     def counters(self, nhalf):
-        out = (Counter * (nhalf * self.nr))()
+        out = np.zeros(nhalf * self.nr, COUNTER_DTYPE)
         lib().ea_fill_counters(self.prc, self.xo, self.nr, self.iter, nhalf,
-                               self.job, out)
+                               self.job, _ptr(out, Counter))
         self.iter += nhalf
         return out
 
@@ -543,7 +558,7 @@ class Simulation:
         pairs = self.pairs(n_it * mesfr) if cluster else np.zeros((1, 4),
                                                                    np.int32)
         sch = Schedule(n_it, mesfr, houfr, ptfr, int(bool(cluster)), it0,
-                       frec_trw, blocks, _ptr(pairs, C.c_int32),
+                       frec_trw, _ptr(blocks, Counter), _ptr(pairs, C.c_int32),
                        self.seed_cl, self.rng_offset)
         sch._keep = (blocks, pairs)
         if cluster:
