@@ -454,11 +454,21 @@ __device__ __forceinline__ void lds_wave_sync() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// Sum over the first `width` lanes of each width-lane group.
+// Sum of e over each W-lane group of the wave (W = 32 or 64), valid in the
+// group's last lane: an inclusive scan within each 16-lane row by DPP row
+// shifts (a lane whose source lies before its row adds 0), then each row's
+// total carried into the next by the row broadcasts (row 0's into row 1 and
+// row 2's into row 3; for W = 64 also lane 31's into rows 2 and 3).  Six
+// VALU instructions and no LDS traffic.
 template <int W>
-__device__ __forceinline__ int group_sum(int e) {
-#pragma unroll
-  for (int off = 1; off < W; off <<= 1) e += __shfl_xor(e, off, 64);
+__device__ __forceinline__ int group_sum_last(int e) {
+  static_assert(W == 32 || W == 64, "groups of 32 or 64 lanes");
+  e += __builtin_amdgcn_update_dpp(0, e, 0x111, 0xf, 0xf, true);  // row_shr:1
+  e += __builtin_amdgcn_update_dpp(0, e, 0x112, 0xf, 0xf, true);  // row_shr:2
+  e += __builtin_amdgcn_update_dpp(0, e, 0x114, 0xf, 0xf, true);  // row_shr:4
+  e += __builtin_amdgcn_update_dpp(0, e, 0x118, 0xf, 0xf, true);  // row_shr:8
+  e += __builtin_amdgcn_update_dpp(0, e, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  if (W == 64) e += __builtin_amdgcn_update_dpp(0, e, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return e;
 }
 
@@ -538,9 +548,9 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *__restrict__ t
     if (ph.energy) {
       // every bond has exactly one odd end: the odd half-sweep sees them
       // all; one partial per (sample, replica, beta, block), no atomics
-      e0 = group_sum<POS>(e0);
-      e1 = group_sum<POS>(e1);
-      if (pos == 0) {
+      e0 = group_sum_last<POS>(e0);
+      e1 = group_sum_last<POS>(e1);
+      if (pos == POS - 1) {
         part[(uint32_t)s0 * pstride + pbase] = e0;
         if (s1 != s0) part[(uint32_t)s1 * pstride + pbase] = e1;
       }
@@ -592,7 +602,7 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_lane(const Tube *__restr
   const int nbx = a.nbx, bx = b.hb * (kLaneThreads / 64) + (tx >> 6);
   const uint32_t pstride = (uint32_t)(g.nr * g.nt * nbx);  // partials of one sample
   const uint32_t pbase = (uint32_t)((r * g.nt + b.beta) * nbx + bx);
-  const bool pwrite = (tx & 63) == 0 && bx < nbx;  // a wave past the last site has none
+  const bool pwrite = (tx & 63) == 63 && bx < nbx;  // a wave past the last site has none
   const size_t sstride = 2 * (size_t)g.half, jstride = 12 * (size_t)g.n;
   const uint32_t jpair = 2 * (uint32_t)g.n;  // words between direction pairs
   for (int s0 = 0; s0 < g.ns; s0 += 2) {
@@ -604,8 +614,8 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_lane(const Tube *__restr
     if (ph.energy) {
       // every bond has exactly one odd end: one partial per (sample,
       // replica, beta, 64 sites), no atomics
-      e0 = group_sum<64>(e0);
-      e1 = group_sum<64>(e1);
+      e0 = group_sum_last<64>(e0);
+      e1 = group_sum_last<64>(e1);
       if (pwrite) {
         part[(uint32_t)s0 * pstride + pbase] = e0;
         if (s1 != s0) part[(uint32_t)s1 * pstride + pbase] = e1;
@@ -691,7 +701,7 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
   const int nbx = a.nbx, bx = b.hb * (kLaneThreads / 64) + (tx >> 6);
   const uint32_t pstride = (uint32_t)(g.nr * g.nt * nbx);
   const uint32_t pbase = (uint32_t)((r * g.nt + b.beta) * nbx + bx);
-  const bool pwrite = (tx & 63) == 0 && bx < nbx;
+  const bool pwrite = (tx & 63) == 63 && bx < nbx;
   const size_t sstride = 2 * (size_t)g.half, jstride = 12 * (size_t)g.n;
   const uint32_t jpair = 2 * (uint32_t)g.n;
 
@@ -750,7 +760,7 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
     stage(buf ^ 1);  // after the last sample: into a buffer nobody reads
     if (valid) S0[c0 + ls.own] = n0;
     if (ph.energy) {
-      e0 = group_sum<64>(e0);
+      e0 = group_sum_last<64>(e0);
       if (pwrite) part[(uint32_t)s0 * pstride + pbase] = e0;
     }
     lds_barrier();
@@ -2057,8 +2067,8 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
       for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
       e0 = 6 * 64 - 2 * u;
     }
-    for (int off = 1; off < 64; off <<= 1) e0 += __shfl_xor(e0, off, 64);
-    if (lane_e == 0 && e0) atomicAdd(esum, e0);
+    e0 = group_sum_last<64>(e0);
+    if (lane_e == 63 && e0) atomicAdd(esum, e0);
   }
   __syncthreads();
   if (tid < 2) {
