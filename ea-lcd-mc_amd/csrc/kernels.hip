@@ -454,14 +454,14 @@ __device__ __forceinline__ void lds_wave_sync() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// Sum of e over each W-lane group of the wave (W = 32 or 64), valid in the
-// group's last lane: an inclusive scan within each 16-lane row by DPP row
-// shifts (a lane whose source lies before its row adds 0), then each row's
-// total carried into the next by the row broadcasts (row 0's into row 1 and
-// row 2's into row 3; for W = 64 also lane 31's into rows 2 and 3).  Six
-// VALU instructions and no LDS traffic.
+// Inclusive prefix sum of e within each W-lane group of the wave (W = 32
+// or 64), so that the group's last lane holds the group's sum: a scan within
+// each 16-lane row by DPP row shifts (a lane whose source lies before its
+// row adds 0), then each row's total carried into the next by the row
+// broadcasts (row 0's into row 1 and row 2's into row 3; for W = 64 also
+// lane 31's into rows 2 and 3).  Six VALU instructions and no LDS traffic.
 template <int W>
-__device__ __forceinline__ int group_sum_last(int e) {
+__device__ __forceinline__ int group_scan(int e) {
   static_assert(W == 32 || W == 64, "groups of 32 or 64 lanes");
   e += __builtin_amdgcn_update_dpp(0, e, 0x111, 0xf, 0xf, true);  // row_shr:1
   e += __builtin_amdgcn_update_dpp(0, e, 0x112, 0xf, 0xf, true);  // row_shr:2
@@ -548,8 +548,8 @@ __global__ void __launch_bounds__(kSwThreads) k_sweep(const Tube *__restrict__ t
     if (ph.energy) {
       // every bond has exactly one odd end: the odd half-sweep sees them
       // all; one partial per (sample, replica, beta, block), no atomics
-      e0 = group_sum_last<POS>(e0);
-      e1 = group_sum_last<POS>(e1);
+      e0 = group_scan<POS>(e0);
+      e1 = group_scan<POS>(e1);
       if (pos == POS - 1) {
         part[(uint32_t)s0 * pstride + pbase] = e0;
         if (s1 != s0) part[(uint32_t)s1 * pstride + pbase] = e1;
@@ -614,8 +614,8 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_lane(const Tube *__restr
     if (ph.energy) {
       // every bond has exactly one odd end: one partial per (sample,
       // replica, beta, 64 sites), no atomics
-      e0 = group_sum_last<64>(e0);
-      e1 = group_sum_last<64>(e1);
+      e0 = group_scan<64>(e0);
+      e1 = group_scan<64>(e1);
       if (pwrite) {
         part[(uint32_t)s0 * pstride + pbase] = e0;
         if (s1 != s0) part[(uint32_t)s1 * pstride + pbase] = e1;
@@ -760,7 +760,7 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
     stage(buf ^ 1);  // after the last sample: into a buffer nobody reads
     if (valid) S0[c0 + ls.own] = n0;
     if (ph.energy) {
-      e0 = group_sum_last<64>(e0);
+      e0 = group_scan<64>(e0);
       if (pwrite) part[(uint32_t)s0 * pstride + pbase] = e0;
     }
     lds_barrier();
@@ -809,9 +809,18 @@ __global__ void __launch_bounds__(256) k_energy(const Tube *__restrict__ tubes, 
   if (lane < g.nr) gv(a.part)[(((size_t)s * g.nr + r) * g.nt + clone) * nw + wv] = e;
 }
 
+// x of another lane of the same quad, by DPP quad_perm (SEL holds the
+// source lane of each quad position, two bits each).
+template <int SEL>
+__device__ __forceinline__ uint64_t quad_swap(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)x, SEL, 0xf, 0xf, false);
+  const uint32_t hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(x >> 32), SEL, 0xf, 0xf, false);
+  return (uint64_t)hi << 32 | lo;
+}
+
+// Sum of v over the wave, in every lane.
 __device__ __forceinline__ int wave_sum(int v) {
-  for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  return __builtin_amdgcn_readlane(group_scan<64>(v), 63);
 }
 
 // Site overlap Q = sum_i s_i(r1) s_i(r2) of replicas r1 < r2, each at the
@@ -1885,11 +1894,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
     }
     const int lane = tid & 63;
     int x = own;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int y = __shfl_up(x, off, 64);
-      if (lane >= off) x += y;
-    }
+    x = group_scan<64>(x);
     if (lane == 63) wtot[tid >> 6] = x;
     __syncthreads();
     int before = 0, total = 0;
@@ -2027,8 +2032,8 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
     const int w = live ? t >> 2 : 0, quarter = t & 3;
     const int p = w >= g.n, h = w - p * g.n;
     uint64_t m = live ? word_move_bits(g, rows, p, h, quarter) : 0;
-    m |= __shfl_xor(m, 1, 64);
-    m |= __shfl_xor(m, 2, 64);
+    m |= quad_swap<0xB1>(m);  // the quad's lanes 1, 0, 3, 2
+    m |= quad_swap<0x4E>(m);  // and 2, 3, 0, 1
     if (live && quarter == 0) {
       const size_t o = (size_t)p * g.half + h;
       const uint64_t qn = qw[w] ^ m;  // q' = q ^ coin(label)
@@ -2067,7 +2072,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
       for (int d = 0; d < 6; ++d) u += __popcll(sv ^ Jw[d] ^ sc[d]);
       e0 = 6 * 64 - 2 * u;
     }
-    e0 = group_sum_last<64>(e0);
+    e0 = group_scan<64>(e0);
     if (lane_e == 63 && e0) atomicAdd(esum, e0);
   }
   __syncthreads();
