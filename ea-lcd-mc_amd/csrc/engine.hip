@@ -956,8 +956,9 @@ extern "C" int ea_run(ea_engine *e, const ea_schedule *sch, uint64_t *rng_offset
   sh.ptfr = sch->ptfr;
   sh.cluster = sch->cluster != 0;
   const Target x = engine_target(e);
+  // the launches carry their step index; the device step counter (read by
+  // replayed batch steps only) is staged afresh by every call
   for (int it = 0; it < sch->n_it; ++it) RC_TRY(enqueue_step(x, sh, it));
-  HIP_TRY(launch_set_step(e->t.step, sch->n_it, e->stream));
   finish_schedule(e, sch, rng_offset_out);
   return EA_OK;
 }

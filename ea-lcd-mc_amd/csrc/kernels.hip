@@ -1204,10 +1204,6 @@ __global__ void k_step_end(const Tube *__restrict__ tubes, int ntubes) {
   if (i < ntubes && *tubes[i].step < tubes[i].nsteps) *tubes[i].step += 1;
 }
 
-__global__ void k_set_step(int *step, int value) {
-  if (threadIdx.x == 0) *step = value;
-}
-
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -2691,10 +2687,6 @@ hipError_t launch_pt(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_set_step(int *step, int value, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, step, value);
-  return hipGetLastError();
-}
 
 hipError_t launch_step_end(const Tube *tubes, int ntubes, hipStream_t s) {
   if (ntubes > 0)

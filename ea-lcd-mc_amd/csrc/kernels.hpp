@@ -212,7 +212,6 @@ hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, int step, hipStre
 hipError_t launch_pt(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
 hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
 hipError_t launch_step_end(const Tube *tubes, int ntubes, hipStream_t s);
-hipError_t launch_set_step(int *step, int value, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
 
