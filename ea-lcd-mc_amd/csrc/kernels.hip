@@ -2519,7 +2519,12 @@ size_t hou_slab_budget(int lds_max, int lds_cu) {
 int hou_slab_planes(const Lattice &g, size_t lds_slab) {
   int planes = std::max(1, std::min(g.Lz, kHouSlabSites / (g.L * g.L)));
   while (planes > 1 && hou_slab_lds(g, planes) > lds_slab) --planes;
-  return planes;
+  // as many slabs as the thickest that fits needs, made as even as they go:
+  // a slab's workgroup takes time in proportion to its planes, and the
+  // thickest one sets the kernel's (Lz = 192 at 60 planes: 48+48+48+48
+  // instead of 60+60+60+12)
+  const int slabs = (g.Lz + planes - 1) / planes;
+  return (g.Lz + slabs - 1) / slabs;
 }
 
 size_t hou_slab_bytes(const Lattice &g, int planes) {
