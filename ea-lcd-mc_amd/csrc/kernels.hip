@@ -1685,6 +1685,11 @@ __device__ __forceinline__ void queue_links(const Lattice &g, const RowT *rows, 
   // takes the top cnt <= 64 pairs; with qn < kHouQueue before, qn stays
   // below kHouQueue (at most 64 come back)
   auto drain = [&](int cnt) {
+    // the drain is a chain of dependent LDS round trips: at a raised wave
+    // priority each step issues as soon as its data is back, ahead of the
+    // other waves' bond pushes (the CU's arbiter otherwise favours the
+    // oldest waves)
+    __builtin_amdgcn_s_setprio(1);
     lds_wave_sync();  // the queue's writes are done
     uint32_t u = 0, v = 0;
     const bool got = lane < cnt;
@@ -1704,6 +1709,7 @@ __device__ __forceinline__ void queue_links(const Lattice &g, const RowT *rows, 
             u | v << 16;
       qn += __popcll(m);
     }
+    __builtin_amdgcn_s_setprio(0);
   };
   auto node_base = [&](int r) { return BY_RUNS ? (uint32_t)rbase[r] : (uint32_t)(r * L); };
   auto node = [&](uint32_t nb, RowT b1, int hbe, int x) {
@@ -1834,6 +1840,12 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   __shared__ int wtot[16];
   __shared__ int nruns;
   const int beta = in.beta, s = in.s, grp = in.grp;
+  // Wave priority 2 until the links: the set-up is a few dependent memory
+  // round trips, and a CU runs up to three of these workgroups, the later
+  // ones arriving while the first link.  Raised, a newcomer's set-up is not
+  // held behind the older workgroups' link instructions, so the three finish
+  // closer together and the last one's end runs less alone.
+  __builtin_amdgcn_s_setprio(2);
   // The instance's two chains: replicas r1, r2 of the call's pairs and their
   // clones at beta (the clones of all replicas are loaded at once, one per
   // lane, beside the pairs), then each chain's energy before the move and
@@ -1939,6 +1951,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   if (by_runs) {
     // two steps of Rem's walk per drain: most unions end within them (1.5
     // to 1.8 steps on average on uncorrelated q), the rest resume later
+    __builtin_amdgcn_s_setprio(0);
     queue_links<true>(g, rows, full, rbase, queue, [&](uint32_t &u, uint32_t &v) {
       return rem_step(rp, u, v) || rem_step(rp, u, v);
     });
@@ -1998,6 +2011,7 @@ __global__ void __launch_bounds__(1024) k_hou_whole(const Tube *__restrict__ tub
   } else {
     for (int i = tid; i < V; i += T) f.p[i] = f.ix(i);  // every site its own root
     __syncthreads();
+    __builtin_amdgcn_s_setprio(0);
     queue_links<false>(g, rows, full, rbase, queue, [&](uint32_t &u, uint32_t &v) {
       f.unite(u, v);
       return true;
