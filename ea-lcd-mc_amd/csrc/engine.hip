@@ -139,6 +139,10 @@ struct ea_engine {
 struct ea_batch {
   int device = 0;
   hipStream_t stream = nullptr;
+  // the captured step's second branch (the whole-tube Houdayer kernels,
+  // beside the slab chain) and the events that fork and join it
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   std::vector<ea_engine *> eng;
   Tube *dt = nullptr;
   DevPlan plan;
@@ -153,6 +157,9 @@ struct ea_batch {
     if (graph) (void)hipGraphDestroy(graph);
     free_plan(&plan);
     if (dt) (void)hipFree(dt);
+    if (join) (void)hipEventDestroy(join);
+    if (fork) (void)hipEventDestroy(fork);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -649,12 +656,17 @@ struct Timed {
 
 // Where the launches of one step go: a tube table, its block maps and a
 // stream; `timer` is the engine whose timing brackets them (NULL in a batch).
+// With a side stream (a batch), a cluster move that has both whole-tube and
+// slab tubes runs the whole-tube kernels there, forked from and joined back
+// to the stream by the two events.
 struct Target {
   const Tube *dt;
   const DevPlan *plan;
   int ntubes;
   hipStream_t stream;
   ea_engine *timer;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 
 // One step of main()'s `it` loop (main.cu:149-228) for every tube of the
@@ -683,7 +695,19 @@ static int enqueue_step(const Target &x, const StepShape &sh, int step = -1) {
     if (sh.cluster) {
       Phase ph{0, 0, im, last_m, step};
       Timed tm(x.timer, EA_K_HOUDAYER);
-      HIP_TRY(launch_houdayer(x.dt, *x.plan, ph, x.stream));
+      if (x.side && hou_has(*x.plan, kHouWhole) && hou_has(*x.plan, kHouSlabs)) {
+        // the two paths touch disjoint tubes: the whole-tube kernels fill
+        // the CUs that the slab chain's last workgroups and its small seam
+        // and energy launches leave idle
+        HIP_TRY(hipEventRecord(x.fork, x.stream));
+        HIP_TRY(hipStreamWaitEvent(x.side, x.fork, 0));
+        HIP_TRY(launch_houdayer(x.dt, *x.plan, ph, x.stream, kHouSlabs));
+        HIP_TRY(launch_houdayer(x.dt, *x.plan, ph, x.side, kHouWhole));
+        HIP_TRY(hipEventRecord(x.join, x.side));
+        HIP_TRY(hipStreamWaitEvent(x.stream, x.join, 0));
+      } else {
+        HIP_TRY(launch_houdayer(x.dt, *x.plan, ph, x.stream));
+      }
     }
   }
   if (step < 0) HIP_TRY(launch_step_end(x.dt, x.ntubes, x.stream));
@@ -1021,6 +1045,9 @@ extern "C" int ea_batch_create(ea_engine *const *engines, int n, ea_batch **out)
   b->eng.assign(engines, engines + n);
   b->device = engines[0]->device;
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
   RC_TRY(dalloc(&b->dt, (size_t)n));
   *out = b.release();
   return EA_OK;
@@ -1091,7 +1118,7 @@ extern "C" int ea_batch_run(ea_batch *b, const ea_schedule *sch, uint64_t *rng_o
   if (!b->exec || !(sh == b->shape)) {
     RC_TRY(drop_graph(b));
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const Target x{b->dt, &b->plan, n, b->stream, nullptr};
+    const Target x{b->dt, &b->plan, n, b->stream, nullptr, b->side, b->fork, b->join};
     const int rc = enqueue_step(x, sh);
     hipGraph_t graph = nullptr;
     const hipError_t ce = hipStreamEndCapture(b->stream, &graph);

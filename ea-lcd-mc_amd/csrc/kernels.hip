@@ -2719,27 +2719,35 @@ hipError_t launch_step_end(const Tube *tubes, int ntubes, hipStream_t s) {
 }
 
 
-hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s) {
+bool hou_has(const DevPlan &p, int part) {
+  return (part == kHouWhole) ? p.cnt[kMapWhole16] || p.cnt[kMapWhole32] || p.cnt[kMapWhole64]
+                             : p.cnt[kMapSlab] != 0;
+}
+
+hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s,
+                           int parts) {
   const dim3 wg(p.shape.cluster_threads);
-  if (p.cnt[kMapWhole16]) {
-    auto k = k_hou_whole<uint32_t, uint16_t>;
-    allow_lds(reinterpret_cast<const void *>(k), p.lds16);
-    hipLaunchKernelGGL(k, dim3((unsigned)p.cnt[kMapWhole16]), wg, p.lds16, s, tubes,
-                       p.map(kMapWhole16), ph);
+  if (parts & kHouWhole) {
+    if (p.cnt[kMapWhole16]) {
+      auto k = k_hou_whole<uint32_t, uint16_t>;
+      allow_lds(reinterpret_cast<const void *>(k), p.lds16);
+      hipLaunchKernelGGL(k, dim3((unsigned)p.cnt[kMapWhole16]), wg, p.lds16, s, tubes,
+                         p.map(kMapWhole16), ph);
+    }
+    if (p.cnt[kMapWhole32]) {
+      auto k = k_hou_whole<uint32_t, uint32_t>;
+      allow_lds(reinterpret_cast<const void *>(k), p.lds32);
+      hipLaunchKernelGGL(k, dim3((unsigned)p.cnt[kMapWhole32]), wg, p.lds32, s, tubes,
+                         p.map(kMapWhole32), ph);
+    }
+    if (p.cnt[kMapWhole64]) {
+      auto k = k_hou_whole<uint64_t, uint32_t>;
+      allow_lds(reinterpret_cast<const void *>(k), p.lds64);
+      hipLaunchKernelGGL(k, dim3((unsigned)p.cnt[kMapWhole64]), wg, p.lds64, s, tubes,
+                         p.map(kMapWhole64), ph);
+    }
   }
-  if (p.cnt[kMapWhole32]) {
-    auto k = k_hou_whole<uint32_t, uint32_t>;
-    allow_lds(reinterpret_cast<const void *>(k), p.lds32);
-    hipLaunchKernelGGL(k, dim3((unsigned)p.cnt[kMapWhole32]), wg, p.lds32, s, tubes,
-                       p.map(kMapWhole32), ph);
-  }
-  if (p.cnt[kMapWhole64]) {
-    auto k = k_hou_whole<uint64_t, uint32_t>;
-    allow_lds(reinterpret_cast<const void *>(k), p.lds64);
-    hipLaunchKernelGGL(k, dim3((unsigned)p.cnt[kMapWhole64]), wg, p.lds64, s, tubes,
-                       p.map(kMapWhole64), ph);
-  }
-  if (p.cnt[kMapSlab]) {
+  if ((parts & kHouSlabs) && p.cnt[kMapSlab]) {
     allow_lds(reinterpret_cast<const void *>(k_hou_slab), p.lds_slab);
     hipLaunchKernelGGL(k_hou_slab, dim3((unsigned)p.cnt[kMapSlab]), dim3(1024), p.lds_slab, s,
                        tubes, p.map(kMapSlab), ph);

@@ -210,7 +210,13 @@ hipError_t launch_energy(const Tube *tubes, const DevPlan &p, int step, hipStrea
 hipError_t launch_overlap(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_pt(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
-hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
+// parts: kHouWhole (the whole-tube kernels), kHouSlabs (the slab chain) or
+// both; the two touch disjoint tubes, so they may run concurrently
+constexpr int kHouWhole = 1, kHouSlabs = 2;
+hipError_t launch_houdayer(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s,
+                           int parts = kHouWhole | kHouSlabs);
+// whether the plan has blocks for part kHouWhole or kHouSlabs
+bool hou_has(const DevPlan &p, int part);
 hipError_t launch_step_end(const Tube *tubes, int ntubes, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *p, double v, size_t n, hipStream_t s);
