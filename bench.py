@@ -58,11 +58,12 @@ def b_alg_sweep(S, nr, nt):
     return 0.375 + 0.375 / S + 0.75 / (nr * nt)
 
 
-def kernel_bytes(kind, w, S, V, ctm):
-    """algorithmic bytes of one launch of each kernel kind"""
+def kernel_bytes(kind, w, S, V, ctm, sweep_halves=1):
+    """algorithmic bytes of one launch of each kernel kind (a sweep launch
+    makes one half-sweep, or both with sweep kind 3)"""
     nr, nt = w["nr"], w["nt"]
     if kind == "sweep":
-        return b_alg_sweep(S, nr, nt) * V * nr * nt * S / 2.0
+        return b_alg_sweep(S, nr, nt) * V * nr * nt * S / 2.0 * sweep_halves
     if kind == "houdayer":
         # q read and both replicas written: 0.5 B per labelled site when the
         # labels stay on chip; 8.5 B when they round-trip HBM
@@ -303,9 +304,10 @@ def measure(name, S, steps, warmup, timing_steps, tune=True, shape=None, tune_st
     tm = eng.read_timing()
     eng.set_timing(False)
     kern = {}
+    halves = 2 if shp[2] == 3 else 1  # half-sweeps per sweep launch
     for k, (ms, n) in tm.items():
         if n:
-            b = kernel_bytes(k, w, S, V, ctm)
+            b = kernel_bytes(k, w, S, V, ctm, halves)
             kern[k] = {"avg_us": 1e3 * ms / n, "launches": n,
                        "share": 0.0, "GBs": b / (1e-3 * ms / n) / 1e9,
                        "frac": b / (1e-3 * ms / n) / 1e9 / HBM_PEAK_GBS}
@@ -317,7 +319,7 @@ def measure(name, S, steps, warmup, timing_steps, tune=True, shape=None, tune_st
     roof = {"bound": "hbm", "kernel": dom, "achieved": ach,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": measured_traffic(name, dom),
-            "bytes_per_launch": kernel_bytes(dom, w, S, V, ctm),
+            "bytes_per_launch": kernel_bytes(dom, w, S, V, ctm, halves),
             "avg_launch_us": kern[dom]["avg_us"]}
     eng.close()
     return {"workload": name, "value": value, "unit": "spin-flip attempts/ns",

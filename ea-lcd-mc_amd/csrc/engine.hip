@@ -127,7 +127,7 @@ struct ea_engine {
     const void *ptrs[] = {t.spin, t.J,    t.wctb, t.lut,  t.ener, t.part,   t.pt,
                           t.acc,  t.att,  t.raw,  t.nd,   t.tab,  t.toff,   t.tlen,
                           t.esum, t.hist, t.hpar, t.hl16, t.hrow, t.hflag,  t.ovl,
-                          keep_buf, call_dev};
+                          t.thr,  keep_buf, call_dev};
     for (const void *p : ptrs)
       if (p) (void)hipFree(const_cast<void *>(p));
     if (call_host) (void)hipHostFree(call_host);
@@ -413,6 +413,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
       (rc = dalloc(&t.acc, chains * (g.nt - 1))) ||
       (rc = dalloc(&t.att, chains * (g.nt - 1))) ||
       (rc = dalloc(&t.raw, chains * (g.nt - 1))) || (rc = dalloc(&t.esum, chains * g.nt)) ||
+      (sweep_full_thr_words(g) && (rc = dalloc(&t.thr, sweep_full_thr_words(g)))) ||
       (rc = grow_call_block(e.get(), kCallInitial))) {
     t.J = J;
     t.lut = lut;
@@ -498,7 +499,7 @@ static uint64_t engine_bytes(const ea_config *cfg, int lds_max, int lds_cu, int 
   uint64_t b = 8ull * g.ns * 2 * g.half + 8ull * g.ns * 12 * g.n + chains * g.nt +
                (uint64_t)g.nt * EA_LUT_BYTES + 4ull * chains * g.nt + sizeof(ea_hq_state) * chains +
                4ull * energy_partials(g) + 24ull * chains * (g.nt - 1) + 8ull * chains * g.nt +
-               kCallInitial;
+               8ull * sweep_full_thr_words(g) + kCallInitial;
   const ExpTables x = make_exp_tables(cfg->betas, g.nt, g.V);
   b += 8ull * x.nd.size() + 8ull * x.off.size() + 8ull * std::max<size_t>(x.val.size(), 1);
   if (g.ctm > 0 && hou_use_slabs(g, lds_max, lds_cu, cus))
@@ -682,12 +683,20 @@ static int enqueue_step(const Target &x, const StepShape &sh, int step = -1) {
   for (int im = 0; im < sh.mesfr; ++im) {
     const bool last_m = im == sh.mesfr - 1;
     for (int ih = 0; ih < sh.houfr; ++ih) {
-      for (int sw = 0; sw < sh.ptfr; ++sw)
+      for (int sw = 0; sw < sh.ptfr; ++sw) {
+        if (sweep_is_whole(x.plan->shape)) {  // both halves in one launch
+          Phase ph{half, sw == sh.ptfr - 1, 0, 0, step};
+          half += 2;
+          Timed tm(x.timer, EA_K_SWEEP);
+          HIP_TRY(launch_sweep(x.dt, *x.plan, ph, x.stream));
+          continue;
+        }
         for (int p = 0; p < 2; ++p) {
           Phase ph{half++, sw == sh.ptfr - 1 && p == 1, 0, 0, step};
           Timed tm(x.timer, EA_K_SWEEP);
           HIP_TRY(launch_sweep(x.dt, *x.plan, ph, x.stream));
         }
+      }
       Phase ph{0, 1, 0, last_m && ih == sh.houfr - 1, step};
       Timed tm(x.timer, EA_K_PT);
       HIP_TRY(launch_pt(x.dt, *x.plan, ph, x.stream));
@@ -734,8 +743,9 @@ static int ensure_energy(ea_engine *e) {
 // Enqueues the 2*nsweeps half-sweeps of the call in flight (timed by
 // ea_set_timing when `timed`).
 static int enqueue_sweeps(ea_engine *e, int nsweeps, bool timed) {
-  for (int half = 0; half < 2 * nsweeps; ++half) {
-    Phase ph{half, half == 2 * nsweeps - 1, 0, 0, 0};
+  const int per = sweep_is_whole(e->plan.shape) ? 2 : 1;  // halves per launch
+  for (int half = 0; half < 2 * nsweeps; half += per) {
+    Phase ph{half, half + per == 2 * nsweeps, 0, 0, 0};
     Timed t(timed ? e : nullptr, EA_K_SWEEP);
     HIP_TRY(launch_sweep(e->dt, e->plan, ph, e->stream));
   }
@@ -853,13 +863,19 @@ static int check_shape(const ea_shape *sh) {
   if (sh->cluster_threads < 256 || sh->cluster_threads > 1024 || sh->cluster_threads % 64)
     return fail(EA_E_ARG, "cluster_threads must be a multiple of 64 in 256..1024 (got %d)",
                 sh->cluster_threads);
-  if (sh->sweep_kind < 0 || sh->sweep_kind > 2)
-    return fail(EA_E_ARG, "sweep_kind must be 0, 1 or 2 (got %d)", sh->sweep_kind);
+  if (sh->sweep_kind < 0 || sh->sweep_kind > 3)
+    return fail(EA_E_ARG, "sweep_kind must be 0, 1, 2 or 3 (got %d)", sh->sweep_kind);
   if (sh->sweep_kind != 0 && sh->sweep_positions != 64)
     return fail(EA_E_ARG, "sweep_kind %d takes sweep_positions 64", sh->sweep_kind);
   if (sh->sweep_xcd != 0 && sh->sweep_xcd != 1)
     return fail(EA_E_ARG, "sweep_xcd must be 0 or 1 (got %d)", sh->sweep_xcd);
   return EA_OK;
+}
+
+// Why a tube does not fit a sweep kind (shape_fits).
+static const char *shape_misfit(int kind) {
+  return kind == 3 ? "the clones of a one-launch sweep need n = L*L*Lz/128 <= 512"
+                   : "the tile (256 + L*L/16 half sites) exceeds 768";
 }
 
 static Shape to_shape(const ea_shape *sh) {
@@ -877,8 +893,7 @@ extern "C" int ea_set_shape(ea_engine *e, const ea_shape *shape) {
   if (!e) return fail(EA_E_ARG, "NULL engine");
   RC_TRY(check_shape(shape));
   if (!shape_fits(e->g(), to_shape(shape)))
-    return fail(EA_E_ARG, "sweep_kind %d: this tube's tile (256 + L*L/16 half sites) exceeds 768",
-                shape->sweep_kind);
+    return fail(EA_E_ARG, "sweep_kind %d: %s", shape->sweep_kind, shape_misfit(shape->sweep_kind));
   if (e->part_pending) {
     // the resolve launch exits unless a call is in flight (a previous
     // ea_run with n_it = 0 leaves none)
@@ -1074,8 +1089,8 @@ extern "C" int ea_batch_set_shape(ea_batch *b, const ea_shape *shape) {
   RC_TRY(check_shape(shape));
   for (size_t i = 0; i < b->eng.size(); ++i)
     if (!shape_fits(b->eng[i]->g(), to_shape(shape)))
-      return fail(EA_E_ARG, "sweep_kind %d: tube %zu's tile (256 + L*L/16 half sites) exceeds 768",
-                  shape->sweep_kind, i);
+      return fail(EA_E_ARG, "sweep_kind %d: tube %zu: %s", shape->sweep_kind, i,
+                  shape_misfit(shape->sweep_kind));
   b->launch = to_shape(shape);
   return EA_OK;
 }

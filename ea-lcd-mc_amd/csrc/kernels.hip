@@ -772,6 +772,180 @@ __global__ void __launch_bounds__(kLaneThreads) k_sweep_tile(const Tube *__restr
   }
 }
 
+// Sample-independent addressing of half site h of parity p within one
+// sample's clone staged in LDS as [2 parities][n]: lane_site with the clone
+// row's own and neighbour words counted from the row's start.
+__device__ __forceinline__ LaneSite lane_site_row(const Lattice &g, int p, int h) {
+  const Nbrs nb = neighbours(g, p, h);
+  LaneSite ls;
+  ls.own = (uint32_t)(p * g.n + h);
+#pragma unroll
+  for (int d = 0; d < 6; ++d) ls.nb[d] = (uint32_t)((1 - p) * g.n + nb.h[d]);
+  ls.jo = (uint32_t)coupling_index(g, 0, p, h, 0);
+  ls.wrap = nb.wrap;
+  ls.top = g.bc == EA_OBC && nb.gz == g.MLz - 1;
+  ls.bottom = g.bc == EA_OBC && nb.gz == 0;
+  return ls;
+}
+
+// One sweep, kind 3: both half-sweeps (ph.half, even, and ph.half + 1) in
+// one launch.  A block of kFullThreads threads holds in LDS the clones at beta of
+// replica r of SG = sweep_full_samples(g) consecutive samples, both parities
+// (16 n bytes each), one thread per half site of each clone.  The even
+// half-sweep updates the clones in LDS and, after a barrier, the odd one
+// reads its neighbours there: each word is loaded and stored once per sweep,
+// and a sweep is one launch instead of two.
+// Random words: the thresholds (id1, id2) of the 2 n positions of (r, beta)
+// for both halves are the same for every sample group.  With G > 1 groups,
+// each launch's group gi makes slice gi of the NEXT sweep's thresholds into
+// a.thr (buffer (q + 1) & 1, q the sweep's index in the call), and takes its
+// own from buffer q & 1, where the previous launch left them; the first
+// sweep of a call (and every sweep of a one-group tube) makes its own, six
+// lanes per word as kind 0 does, with the half's counter block.  The words
+// depend on the counters and beta only, so either way they equal the
+// reference's.  Each thread's couplings of both halves, its thresholds and
+// its clone row are loaded first, then its 16-byte chunk of the clones,
+// whose latency overlaps the random words.  Blocks (sample group, replica,
+// beta), beta fastest; one energy partial per (sample, replica, beta)
+// (nbx = 1) when ph.energy.
+template <bool XCD>
+__global__ void __launch_bounds__(kFullThreads) k_sweep_full(const Tube *__restrict__ tubes,
+                                                           const uint64_t *__restrict__ map,
+                                                           Phase ph) {
+  __shared__ __align__(16) uint32_t lut[EA_LUT_BYTES / 4];
+  __shared__ uint32_t ctr[4][5];  // this sweep's two halves, then the next sweep's
+  __shared__ int esum[EA_MAX_SAMPLES];
+  extern __shared__ uint64_t dyn[];
+  const Where wh = locate_block(tubes, map, ph.step,
+                                XCD ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int n = g.n, tx = threadIdx.x, T = blockDim.x;
+  const int rest = wh.local / g.nt, beta = wh.local - rest * g.nt;
+  const int r = rest & (g.nr - 1), gi = rest >> g.nr_shift, SG = sweep_full_samples(g);
+  const int G = (g.ns + SG - 1) / SG, s0 = gi * SG, sgn = min(SG, g.ns - s0);
+  const int hidx = wh.step * a.halves + ph.half, q = hidx >> 1;  // the call's half, sweep
+  const bool own = G == 1 || q == 0;
+  const bool next = G > 1 && hidx + 2 < a.nsteps * a.halves;
+  const int nlo = gi * 2 * n / G, nhi = (gi + 1) * 2 * n / G;  // slice of the next sweep
+  const int town = own ? 12 * n : 0, tall = town + (next ? 6 * (nhi - nlo) : 0);
+  uint64_t *sp = dyn;                                           // [SG][2][n]
+  uint64_t *ids = sp + (size_t)2 * SG * n;                      // [2][n][2]
+  uint32_t *words = reinterpret_cast<uint32_t *>(ids + 4 * n);  // own [2n][6], next slice
+  const size_t tbuf = (size_t)g.nr * g.nt * 4 * n, trow = ((size_t)r * g.nt + beta) * 4 * n;
+  // Thread tx: half site h of sample s0 + j in both halves, and the clones'
+  // 16-byte chunk tx (words 2 tx, 2 tx + 1 of sp: half sites ch, ch + 1 of
+  // parity cp of the same sample).  Idle threads read sample s0's first
+  // chunk, couplings and thresholds, and write nothing.
+  const bool act = tx < sgn * n;
+  const int j = act ? tx / n : 0, h = act ? tx - j * n : 0;
+  const int cp = 2 * h >= n, ch = 2 * h - cp * n;
+  if (own || next) {
+    const auto src = gv(reinterpret_cast<const uint4 *>(a.lut + (size_t)beta * EA_LUT_BYTES));
+    const auto vsrc = reinterpret_cast<const global_t<u32x4> *>(src);
+    u32x4 *dst = reinterpret_cast<u32x4 *>(lut);
+    for (int i = tx; i < EA_LUT_BYTES / 16; i += T) dst[i] = vsrc[i];
+  }
+  if (tx < 4 && (tx < 2 ? own : next)) {
+    const auto c = gv(a.ctr) + ((size_t)hidx + tx) * g.nr + r;
+    ctr[tx][0] = c->x;
+    ctr[tx][1] = c->y;
+    ctr[tx][2] = c->z;
+    ctr[tx][3] = c->w;
+    ctr[tx][4] = a.keys[r];
+  }
+  if (tx < sgn) esum[tx] = 0;
+  const int clone = gv(a.wctb)[((size_t)(s0 + j) * g.nr + r) * g.nt + beta];
+  uint64_t Jv[2][6];
+  {
+    const gword *Jb = gmem(a.J + (size_t)(s0 + j) * 12 * n);
+    const uint32_t jpair = 2 * (uint32_t)n;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const uint32_t jo = (uint32_t)coupling_index(g, 0, p, h, 0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        Jv[p][2 * k] = Jb[jo + k * jpair];
+        Jv[p][2 * k + 1] = Jb[jo + k * jpair + 1];
+      }
+    }
+  }
+  u64x2 idv[2] = {u64x2{0, 0}, u64x2{0, 0}};
+  if (!own) {  // made by the previous launch
+    const auto cur = reinterpret_cast<const global_t<u64x2> *>(gmem(a.thr + (q & 1) * tbuf + trow));
+    idv[0] = cur[h];
+    idv[1] = cur[n + h];
+  }
+  // the chunk's words (row offsets and n are even: 16-byte aligned)
+  const auto chunk = reinterpret_cast<global_t<u64x2> *>(
+      gmem(a.spin + (size_t)(s0 + j) * 2 * g.half + (size_t)cp * g.half) +
+      spin_row(g, r, clone) + ch);
+  const u64x2 cv = *chunk;
+  if (tall) {
+    lds_barrier();  // LUT and counters staged; the other loads stay in flight
+    for (int t = tx; t < tall; t += T) {  // word k of position pos = p n + h'
+      const bool mine = t < town;
+      const int tt = mine ? t : t - town, pq = tt / 6, k = tt - 6 * pq;
+      const int pos = mine ? pq : nlo + pq, p = pos >= n, hp = pos - p * n;
+      const uint32_t(&c)[5] = ctr[(mine ? 0 : 2) + p];
+      const uint32_t tid = ((uint32_t)beta * (uint32_t)n + (uint32_t)hp) * (uint32_t)g.nr + r;
+      words[t] = rnd_word(c[0] + k, c[1], c[2] + tid, c[3], tid, c[4], lut);
+    }
+    lds_barrier();
+    const auto nxt = reinterpret_cast<global_t<u64x2> *>(gmem(a.thr + ((q + 1) & 1) * tbuf + trow));
+    for (int t = tx; 6 * t < tall; t += T) {
+      uint64_t i1, i2;
+      thresholds(words + 6 * t, i1, i2);
+      if (6 * t < town) {
+        ids[2 * t] = i1;
+        ids[2 * t + 1] = i2;
+      } else {
+        nxt[nlo + t - 2 * n * own] = u64x2{i1, i2};
+      }
+    }
+  }
+  if (act) *reinterpret_cast<u64x2 *>(sp + 2 * tx) = cv;
+  lds_barrier();
+  if (own) {
+    idv[0] = u64x2{ids[2 * h], ids[2 * h + 1]};
+    idv[1] = u64x2{ids[2 * (n + h)], ids[2 * (n + h) + 1]};
+  }
+  uint64_t *S = sp + (size_t)j * 2 * n;  // this thread's sample's clone
+  int e = 0;
+  // the two halves: parity p reads the other parity's words in LDS
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    if (act) {
+      const LaneSite ls = lane_site_row(g, p, h);
+      SiteIn x;
+      x.sv = S[ls.own];
+#pragma unroll
+      for (int d = 0; d < 6; ++d) x.sc[d] = S[ls.nb[d]];
+#pragma unroll
+      for (int d = 0; d < 6; ++d) x.J[d] = Jv[p][d];
+      fix_lane(ls, p, x);
+      const uint64_t f = site_flip(x, idv[p][0], idv[p][1], p == 1 && ph.energy, e);
+      S[ls.own] = x.sv ^ f;
+    }
+    lds_barrier();
+  }
+  if (ph.energy) {
+    // every bond has exactly one odd end: the odd half's words hold them all
+    if ((n & 63) == 0) {  // each wave's lanes belong to one sample
+      e = group_scan<64>(e);
+      if ((tx & 63) == 63 && act) atomicAdd(&esum[j], e);
+    } else if (act) {
+      atomicAdd(&esum[j], e);
+    }
+    lds_barrier();
+    if (tx < sgn)
+      gv(a.part)[(uint32_t)(s0 + tx) * (uint32_t)(g.nr * g.nt) + (uint32_t)(r * g.nt + beta)] =
+          esum[tx];
+  }
+  if (act) *chunk = *reinterpret_cast<const u64x2 *>(sp + 2 * tx);
+}
+
 // ----------------------------------------------------------------- energy
 
 // Energy of every (sample, replica, clone) from the even words
@@ -2482,15 +2656,19 @@ inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) /
 // ------------------------------------------------------------- launchers
 
 int sweep_partials_x(const Lattice &g, const Shape &shape) {
+  if (shape.sweep_kind == 3) return 1;
   return (int)blocks_for(g.n, shape.sweep_kind ? 64 : shape.sweep_positions);
 }
 
 bool shape_fits(const Lattice &g, const Shape &shape) {
+  if (shape.sweep_kind == 3) return g.n <= kFullMaxN;
   return shape.sweep_kind != 2 || tile_words(g) <= kTileWordsMax;
 }
 
 static long long sweep_grid(const Tube &t, const Shape &shape) {
   const Lattice &g = t.g;
+  if (shape.sweep_kind == 3)
+    return blocks_for(g.ns, sweep_full_samples(g)) * g.nr * g.nt;
   const long long per = shape.sweep_kind ? blocks_for(g.n, kLaneThreads) : t.nbx;
   return per * g.nr * g.nt;
 }
@@ -2566,6 +2744,7 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
     };
     add(kMapSweep, sweep_grid(t, shape));
     if (shape.sweep_kind == 2) p.tile_words = std::max(p.tile_words, tile_chunks(g));
+    if (shape.sweep_kind == 3) p.lds_full = std::max(p.lds_full, sweep_full_lds(g));
     add(kMapEnergy, (long long)energy_blocks_host(g) * g.nt * g.ns);
     add(kMapChain, (long long)g.ns * g.nr);
     add(kMapOverlap, (long long)g.ns * (g.nr * (g.nr - 1) / 2) * g.nt);
@@ -2610,6 +2789,7 @@ hipError_t upload_plan(const Plan &p, DevPlan *d) {
   d->lds_slab = p.lds_slab;
   d->lds_apply = p.lds_apply;
   d->lds_seam = p.lds_seam;
+  d->lds_full = p.lds_full;
   d->tile_words = p.tile_words;
   d->shape = p.shape;
   d->identity = p.ntubes == 1;
@@ -2653,7 +2833,11 @@ hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream
   const dim3 grid((unsigned)p.cnt[kMapSweep]);
   const uint64_t *m = p.map(kMapSweep);
   const Shape &sh = p.shape;
-  if (sh.sweep_kind == 2) {
+  if (sh.sweep_kind == 3) {
+    auto k = sh.sweep_xcd ? k_sweep_full<true> : k_sweep_full<false>;
+    allow_lds(reinterpret_cast<const void *>(k), p.lds_full);
+    hipLaunchKernelGGL(k, grid, dim3(kFullThreads), p.lds_full, s, tubes, m, ph);
+  } else if (sh.sweep_kind == 2) {
     const dim3 blk(kLaneThreads);
     if (p.tile_words <= 1)
       hipLaunchKernelGGL((sh.sweep_xcd ? k_sweep_tile<1, true> : k_sweep_tile<1, false>), grid,

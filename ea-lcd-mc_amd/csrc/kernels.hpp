@@ -111,6 +111,10 @@ struct Tube {
   uint16_t *hl16;
   uint64_t *hrow, *hflag;
   uint32_t *keep;           // copy of the last labels, or NULL
+  // sweep kind 3: the next sweep's thresholds (id1, id2) of every position,
+  // [2 buffers][nr][nt][2 parities][n][2], made a slice per sample group by
+  // the sweep before (NULL when n > kFullMaxN)
+  uint64_t *thr;
   int planes;               // slab Houdayer planes; 0: whole tube in one block
   int hou_cap;              // run nodes of the whole-tube labelling (hou_run_cap)
   int nbx;                  // sweep blocks per beta
@@ -151,7 +155,9 @@ enum MapKind {
 // among the samples through LDS (1024-thread blocks of sweep_positions = 32
 // or 64 half sites of one replica), kind 1 gives each thread one position
 // and all samples (256-thread blocks, one energy partial per wave);
-// kind 2 is kind 1 with the neighbour words staged in LDS tiles;
+// kind 2 is kind 1 with the neighbour words staged in LDS tiles; kind 3
+// makes a whole sweep (both halves) in one launch, each block holding whole
+// clones in LDS (tubes with n <= kFullMaxN);
 // sweep_xcd = 1 hands each XCD a contiguous range of sweep blocks.  cluster_threads: threads of
 // a whole-tube Houdayer workgroup (256, 512 or 1024).
 struct Shape {
@@ -166,10 +172,32 @@ struct Shape {
 };
 constexpr int kSweepPosMin = 32;
 
+// Kind 3 (one launch per sweep): a block of kFullThreads threads holds the
+// clones at one beta of one replica of sweep_full_samples(g) samples, both
+// parities, one thread per half site of each; tubes with at most kFullMaxN
+// half sites per parity take it.
+constexpr int kFullThreads = 512;
+constexpr int kFullMaxN = 512;
+__host__ __device__ inline int sweep_full_samples(const Lattice &g) {
+  const int sg = kFullThreads / g.n;
+  return sg < 1 ? 1 : (sg < g.ns ? sg : g.ns);
+}
+// its dynamic LDS: clones [SG][2][n] u64 | thresholds [2][n][2] u64 |
+// random words: the sweep's [2][n][6] u32 and a slice of the next sweep's
+// (at most [n][6] with two or more sample groups)
+__host__ __device__ inline size_t sweep_full_lds(const Lattice &g) {
+  return (size_t)sweep_full_samples(g) * 16 * g.n + 32 * (size_t)g.n + 72 * (size_t)g.n;
+}
+// words of Tube::thr: two buffers of [nr][nt][2n][2] thresholds
+inline size_t sweep_full_thr_words(const Lattice &g) {
+  return g.n <= kFullMaxN ? (size_t)8 * g.nr * g.nt * g.n : 0;
+}
+
 struct Plan {
   std::vector<uint64_t> map[kMapKinds];
   Shape shape;
   size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0, lds_seam = 0;
+  size_t lds_full = 0;  // dynamic LDS of a kind-3 sweep (the largest tube's)
   int tile_words = 0;  // 16-byte tile chunks per thread of a tiled sweep (the widest tube's)
   int ntubes = 0;
 };
@@ -182,6 +210,7 @@ struct DevPlan {
   size_t off[kMapKinds] = {};
   size_t cnt[kMapKinds] = {};
   size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0, lds_seam = 0;
+  size_t lds_full = 0;
   int tile_words = 0;  // 16-byte tile chunks per thread of a tiled sweep (the widest tube's)
   Shape shape;
   bool identity = false;
@@ -193,7 +222,7 @@ struct DevPlan {
 // (a tube's nbx), and the partial-energy slots a tube needs under any shape.
 int sweep_partials_x(const Lattice &g, const Shape &shape);
 // Whether a tube can take a launch shape (the tiled sweep needs its tile,
-// 256 + 2 hs half sites, within 768).
+// 256 + 2 hs half sites, within 768; the one-launch sweep n <= kFullMaxN).
 bool shape_fits(const Lattice &g, const Shape &shape);
 size_t energy_partials(const Lattice &g);
 
@@ -205,7 +234,10 @@ hipError_t upload_plan(const Plan &p, DevPlan *d);
 void free_plan(DevPlan *d);
 
 // Launches over a device tube table.  Each returns hipGetLastError().
+// One half-sweep (ph.half), or with kind 3 (sweep_is_whole) the whole sweep
+// of halves ph.half (even) and ph.half + 1.
 hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
+inline bool sweep_is_whole(const Shape &sh) { return sh.sweep_kind == 3; }
 hipError_t launch_energy(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_overlap(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);

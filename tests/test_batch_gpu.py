@@ -180,7 +180,7 @@ def test_launch_shapes_with_other_replica_counts(nr):
         return state(x)
 
     want = sim(None)
-    for shp in ((32, 512, 0, 1), (64, 1024, 1, 0), (64, 256, 2, 1)):
+    for shp in ((32, 512, 0, 1), (64, 1024, 1, 0), (64, 256, 2, 1), (64, 512, 3, 0)):
         say("nr", nr, "shape", shp)
         same(want, sim(shp), "nr %d shape %s" % (nr, shp))
 

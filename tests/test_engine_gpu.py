@@ -480,6 +480,30 @@ def test_shape_change_after_an_empty_run_sums_pending_energies():
     assert np.array_equal(e.engine.energy().ravel(), o.field("ener"))
 
 
+@pytest.mark.parametrize("L,Lz,nt,ns,bc,ptfr", [(16, 64, 6, 31, 1, 1), (16, 64, 4, 5, 0, 2),
+                                                  (8, 16, 5, 31, 0, 1), (24, 24, 4, 3, 1, 2),
+                                                  (16, 128, 3, 2, 0, 1)])
+def test_one_launch_sweeps_match_oracle(L, Lz, nt, ns, bc, ptfr):
+    """Sweep kind 3 (both half-sweeps of whole clones in one launch, the
+    clones staged in LDS) against the oracle, through ea_run with cluster
+    moves and through ea_sweep: n = 128 (the headline tube: 31 samples in 8
+    groups of 4, so each launch after a call's first takes its random words
+    from the launch before), 8 (31 samples in one group, which makes its
+    own), 108 (not a multiple of 64: per-lane energy sums) and 256 (two
+    samples per group)."""
+    o, e = pair(L, Lz, 4, nt, ns, bc=bc, seeds=(61, 62, 63, 64 + bc))
+    e.engine.set_shape(64, 512, 3, 1)
+    kw = dict(ptfr=ptfr, cluster=True)
+    e.steps(3, it0=0, frec_trw=1, **kw)
+    o.run(3, it0=0, frec_trw=1, rows=3, **kw)
+    assert_same_state(o, e, "kind 3 run")
+    e.engine.sweep(e.counters(4))  # two sweeps
+    o.sweeps(2)
+    o.energy()
+    assert np.array_equal(e.spins(), o.spins()), "kind 3 ea_sweep spins"
+    assert np.array_equal(e.engine.energy().ravel(), o.field("ener")), "kind 3 energies"
+
+
 @pytest.mark.parametrize("ptfr,houfr,mesfr", [(2, 1, 1), (1, 2, 1), (1, 1, 2),
                                               (2, 2, 2)])
 @pytest.mark.parametrize("bc", [0, 1])
