@@ -841,21 +841,27 @@ __global__ void __launch_bounds__(kFullThreads) k_sweep_full(const Tube *__restr
   const bool act = tx < sgn * n;
   const int j = act ? tx / n : 0, h = act ? tx - j * n : 0;
   const int cp = 2 * h >= n, ch = 2 * h - cp * n;
-  if (own || next) {
+  // Every load is issued first, into registers (the clone row's chunk after
+  // its row offset): the block waits for one memory round trip, then two
+  // dependent ones, before its LDS stores, instead of one per array.
+  constexpr int kLutChunks = (EA_LUT_BYTES / 16 + kFullThreads - 1) / kFullThreads;
+  const bool makes = own || next;  // random words in this launch
+  u32x4 lv[kLutChunks];
+  if (makes) {
     const auto src = gv(reinterpret_cast<const uint4 *>(a.lut + (size_t)beta * EA_LUT_BYTES));
     const auto vsrc = reinterpret_cast<const global_t<u32x4> *>(src);
-    u32x4 *dst = reinterpret_cast<u32x4 *>(lut);
-    for (int i = tx; i < EA_LUT_BYTES / 16; i += T) dst[i] = vsrc[i];
+#pragma unroll
+    for (int i = 0; i < kLutChunks; ++i) lv[i] = vsrc[min(tx + i * T, EA_LUT_BYTES / 16 - 1)];
   }
-  if (tx < 4 && (tx < 2 ? own : next)) {
+  uint32_t cw[4] = {0, 0, 0, 0};
+  const bool cnt = tx < 4 && (tx < 2 ? own : next);
+  if (cnt) {
     const auto c = gv(a.ctr) + ((size_t)hidx + tx) * g.nr + r;
-    ctr[tx][0] = c->x;
-    ctr[tx][1] = c->y;
-    ctr[tx][2] = c->z;
-    ctr[tx][3] = c->w;
-    ctr[tx][4] = a.keys[r];
+    cw[0] = c->x;
+    cw[1] = c->y;
+    cw[2] = c->z;
+    cw[3] = c->w;
   }
-  if (tx < sgn) esum[tx] = 0;
   const int clone = gv(a.wctb)[((size_t)(s0 + j) * g.nr + r) * g.nt + beta];
   uint64_t Jv[2][6];
   {
@@ -882,9 +888,24 @@ __global__ void __launch_bounds__(kFullThreads) k_sweep_full(const Tube *__restr
       gmem(a.spin + (size_t)(s0 + j) * 2 * g.half + (size_t)cp * g.half) +
       spin_row(g, r, clone) + ch);
   const u64x2 cv = *chunk;
-  if (tall) {
-    lds_barrier();  // LUT and counters staged; the other loads stay in flight
-    for (int t = tx; t < tall; t += T) {  // word k of position pos = p n + h'
+  if (makes) {
+    u32x4 *dst = reinterpret_cast<u32x4 *>(lut);
+#pragma unroll
+    for (int i = 0; i < kLutChunks; ++i)
+      if (tx + i * T < EA_LUT_BYTES / 16) dst[tx + i * T] = lv[i];
+  }
+  if (cnt) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ctr[tx][k] = cw[k];
+    ctr[tx][4] = a.keys[r];
+  }
+  if (tx < sgn) esum[tx] = 0;
+  // Words k of positions pos = p n + h' of this sweep (own: task t < town)
+  // and of the next sweep's slice [nlo, nhi) (t >= town), then their
+  // thresholds: own ones into ids, the next sweep's into a.thr.
+  auto make_words = [&]() {
+    lds_barrier();  // LUT and counters staged
+    for (int t = tx; t < tall; t += T) {
       const bool mine = t < town;
       const int tt = mine ? t : t - town, pq = tt / 6, k = tt - 6 * pq;
       const int pos = mine ? pq : nlo + pq, p = pos >= n, hp = pos - p * n;
@@ -904,7 +925,11 @@ __global__ void __launch_bounds__(kFullThreads) k_sweep_full(const Tube *__restr
         nxt[nlo + t - 2 * n * own] = u64x2{i1, i2};
       }
     }
-  }
+  };
+  // a launch that makes its own words makes them before the update; one
+  // that only makes the next sweep's slice makes it after its stores are
+  // issued, so that the slice overlaps their latency
+  if (own) make_words();
   if (act) *reinterpret_cast<u64x2 *>(sp + 2 * tx) = cv;
   lds_barrier();
   if (own) {
@@ -944,6 +969,7 @@ __global__ void __launch_bounds__(kFullThreads) k_sweep_full(const Tube *__restr
           esum[tx];
   }
   if (act) *chunk = *reinterpret_cast<const u64x2 *>(sp + 2 * tx);
+  if (!own && next) make_words();
 }
 
 // ----------------------------------------------------------------- energy
