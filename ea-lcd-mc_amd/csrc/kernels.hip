@@ -1131,6 +1131,19 @@ __device__ __forceinline__ double exact_exp(const Tube &a, int i, int d) {
   return k <= a.tlen[i] ? a.tab[a.toff[i] + k - 1] : 0.0;
 }
 
+// f^m(z) for the congruential step f(z) = kLcg z + 1 (mod 2^64) of the
+// Parisi-Rapuano generator (random_generator.h:133): the affine map is
+// squared (A, C) -> (A A, A C + C) and applied for each set bit of m.
+__device__ __forceinline__ uint64_t lcg_jump(uint64_t z, uint32_t m) {
+  uint64_t A = kLcg, C = 1;
+  for (; m; m >>= 1) {
+    if (m & 1u) z = A * z + C;
+    C = A * C + C;
+    A = A * A;
+  }
+  return z;
+}
+
 // Parallel_Tempering (upd.c:3-48), one wave per (sample, replica) chain.
 // The chain's Parisi-Rapuano state is staged in LDS.  Its draws do not
 // depend on the swap decisions, so the wave makes them first: with lags 24,
@@ -1141,7 +1154,9 @@ __device__ __forceinline__ double exact_exp(const Tube &a, int i, int d) {
 // comparison u < exp(pt) is made with the host libm's exp from the table.
 // Both sides' errors are far below 1e-12, so every decision equals the
 // reference's.  The statistics are then added by one lane per pair with the
-// tabulated exp values.
+// tabulated exp values.  The block is one wave and its barriers wait for
+// LDS traffic only (lds_barrier): __syncthreads would also wait for every
+// global store in flight, such as the generator state's, in mid-walk.
 __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const uint64_t *__restrict__ map, Phase ph) {
   __shared__ uint64_t ira[256];
   __shared__ uint64_t zs[EA_MAX_BETAS], prs[EA_MAX_BETAS];
@@ -1224,20 +1239,19 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
       sraw[i] = r_raw[j];
     }
   }
-  __syncthreads();
+  lds_barrier();
   if (energy) {
     for (int b = lane; b < nt; b += 64) {
       E[perm[b]] = dE[b];
       ener[perm[b]] = dE[b];
     }
-    __syncthreads();
+    lds_barrier();
   }
   for (int p = lane; p < nt; p += 64) Ep[p] = E[perm[p]];
-  if (lane == 0) {  // the congruential half is a cheap serial recurrence
-    uint64_t z = zseed;
-    for (int k = 0; k < np; ++k) zs[k] = z = z * kLcg + 1;
-  }
-  __syncthreads();
+  // the congruential half: zs[k] = f^(k+1)(zseed) with f(z) = kLcg z + 1,
+  // every k at once by squaring the affine map (lcg_jump)
+  for (int k = lane; k < np; k += 64) zs[k] = lcg_jump(zseed, (uint32_t)k + 1);
+  lds_barrier();
   // a ring slot is rewritten 256 draws after it was written, and every
   // value a draw reads was made 24 to 61 draws before it: each round reads
   // only slots of earlier rounds that no later draw has touched yet
@@ -1249,12 +1263,12 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
       nv = ira[(uint8_t)(ip + k - 24)] + ira[(uint8_t)(ip + k - 55)];
       pr = nv ^ ira[(uint8_t)(ip + k - 61)];
     }
-    __syncthreads();
+    lds_barrier();
     if (act) {
       ira[(uint8_t)(ip + k)] = nv;
       prs[k] = pr;
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (lane == 0) exact_walk = a.pt_exact;
   for (int k = lane; k < np; k += 64) {
@@ -1262,13 +1276,8 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
     const double u = kFnorm * (double)fin;
     uu[k] = u;
     lu[k] = log(u);
-    if (k == np - 1) {
-      st->final_ = fin;
-      st->PR = prs[k];
-      st->zseed = zs[k];
-    }
   }
-  __syncthreads();
+  lds_barrier();
   // Each pair's decision is monotone in its (even) energy step d: pt grows
   // with d, and both the log comparison and the exact one against the
   // table of exp values accept more as pt grows.  So pair i swaps iff
@@ -1290,15 +1299,21 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
     } else if (uu[k] == 0.0) {
       dm = -2 * stlen[k];  // u = 0 < exp(pt) while the table has a value
     } else {
-      const double x = lu[k] / ndv[k];  // the rule turns near d = x < 0
-      dm = x < -1e9 ? INT_MIN : 2 * (int)floor(x * 0.5) - 4;
+      // the rule turns near d = lu / nd < 0: the search starts at the even
+      // d at or below an estimate of it from the approximate reciprocal and
+      // steps up until the rule accepts.  The check below catches a start
+      // above the threshold or too far below it (an estimate off by more
+      // than a few steps, possible only for |d| far beyond the tube's
+      // energy steps) and sends the walk to the exact rule.
+      const double x = lu[k] * __builtin_amdgcn_rcp(ndv[k]);
+      dm = x < -1e9 ? INT_MIN : 2 * (int)floor(x * 0.5);
       for (int t = 0; t < 8 && dm != INT_MIN && !take(dm); ++t) dm += 2;
     }
     if (dm > 0) dm = 0;
     if (dm != INT_MIN && (!take(dm) || take(dm - 2))) exact_walk = 1;
     dmin[k] = dm;
   }
-  __syncthreads();
+  lds_barrier();
   if (!exact_walk) {
     // every decision is d >= dmin[i]: the walk runs in scalar registers, 64
     // pairs at a time.  Lane k holds pair base+k's clone and energy above it
@@ -1311,30 +1326,55 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
       const int i = base + lane, cnt = min(64, np - base);
       const int vpb = i < np ? perm[i + 1] : 0;
       const int32_t veb = i < np ? Ep[i + 1] : 0, vdm = i < np ? dmin[i] : 0;
-      int32_t myd = 0;
-      int mytake = 0, mykeep = 0;
-      for (int k = 0; k < cnt; ++k) {
-        const int pb = __builtin_amdgcn_readlane(vpb, k);
-        const int32_t eb = __builtin_amdgcn_readlane(veb, k);
-        const int d = eb - ec;
-        const bool take = d >= __builtin_amdgcn_readlane(vdm, k);
-        if (lane == k) {
-          myd = d;
-          mytake = take;
-          mykeep = take ? pb : pc;  // pc moves to position i+1 when taken
+      // the serial pass keeps only the energy carried up the ladder and the
+      // take bits, in scalar registers; the readlanes do not depend on it,
+      // so eight pairs' operands are read first and then stepped through
+      // (a loop step per pair, readlane to compare to branch, took about
+      // 110 cycles)
+      uint64_t takes = 0;
+      int32_t e = ec;
+      for (int k0 = 0; k0 < cnt; k0 += 8) {  // k0 + 7 <= 63
+        int32_t eb[8], dm[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          eb[u] = __builtin_amdgcn_readlane(veb, k0 + u);
+          dm[u] = __builtin_amdgcn_readlane(vdm, k0 + u);
         }
-        if (!take) {
-          pc = pb;
-          ec = eb;
+        // (steps past the chunk's last pair read lanes holding zeros: their
+        // bits are masked off below, and the energy they carry is not used)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const bool t = eb[u] - e >= dm[u];
+          const uint64_t bit = 1ull << (k0 + u);
+          takes = t ? takes | bit : takes;  // selects: the step stays scalar
+          e = t ? e : eb[u];
         }
       }
-      __syncthreads();  // the chunk's reads of perm are done
+      // each pair's outcome from the bits: the clone that meets position
+      // i + 1 is the one carried from the last refusal j < i of the chunk
+      // (position j + 1's clone and energy, lane j's), or, with none, the one
+      // carried into the chunk (pc, ec)
+      const uint64_t refused = ~takes & (cnt == 64 ? ~0ull : (1ull << cnt) - 1);
+      const uint64_t prior = refused & ((1ull << lane) - 1);
+      const int j = prior ? 63 - __clzll((long long)prior) : 0;
+      const int cj = __shfl(vpb, j), ej = __shfl(veb, j);
+      const int pcl = prior ? cj : pc;
+      const int32_t ecl = prior ? ej : ec;
+      const bool take = (takes >> lane) & 1ull;
+      const int32_t myd = veb - ecl;
+      const int mykeep = take ? vpb : pcl;  // the carried clone moves on when taken
+      if (refused) {
+        const int jl = 63 - __clzll((long long)refused);
+        pc = __builtin_amdgcn_readlane(vpb, jl);
+        ec = __builtin_amdgcn_readlane(veb, jl);
+      }
+      lds_barrier();  // the chunk's reads of perm are done
       if (i < np) {
         dE[i] = myd;
-        took[i] = (uint8_t)mytake;
+        took[i] = (uint8_t)take;
         perm[i] = (uint8_t)mykeep;
       }
-      __syncthreads();
+      lds_barrier();
     }
     if (lane == 0) perm[np] = (uint8_t)pc;
   } else if (lane == 0) {
@@ -1367,13 +1407,16 @@ __global__ void __launch_bounds__(64) k_pt(const Tube *__restrict__ tubes, const
     }
     perm[np] = (uint8_t)pc;
   }
-  if (lane == 0) {
+  if (lane == 0) {  // the generator's state after the call's last draw
+    st->final_ = zs[np - 1] + prs[np - 1];
+    st->PR = prs[np - 1];
+    st->zseed = zs[np - 1];
     st->ip = (uint8_t)(ip + np);
     st->ip1 = (uint8_t)(ip1 + np);
     st->ip2 = (uint8_t)(ip2 + np);
     st->ip3 = (uint8_t)(ip3 + np);
   }
-  __syncthreads();
+  lds_barrier();
   for (int i = lane; i < 256; i += 64) st->ira[i] = ira[i];
   for (int i = lane; i < nt; i += 64) wctb[i] = perm[i];
   if (measure) {  // energy sums, history (main.cu:208-228)
