@@ -14,6 +14,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "host.hpp"
@@ -127,7 +128,7 @@ struct ea_engine {
     const void *ptrs[] = {t.spin, t.J,    t.wctb, t.lut,  t.ener, t.part,   t.pt,
                           t.acc,  t.att,  t.raw,  t.nd,   t.tab,  t.toff,   t.tlen,
                           t.esum, t.hist, t.hpar, t.hl16, t.hrow, t.hflag,  t.ovl,
-                          t.thr,  keep_buf, call_dev};
+                          t.thr,  t.spin2, keep_buf, call_dev};
     for (const void *p : ptrs)
       if (p) (void)hipFree(const_cast<void *>(p));
     if (call_host) (void)hipHostFree(call_host);
@@ -393,6 +394,7 @@ extern "C" int ea_create(const ea_config *cfg, ea_engine **out) {
   g.half = (int64_t)g.n * g.nr * g.nt;
   set_divisors(g);
   t.nbx = sweep_partials_x(g, e->shape);
+  t.sweep_planes = sweep_slab_planes(g);
   t.frec = 1;
   t.cluster = g.ctm > 0 && g.nr >= 4;
   t.pt_exact = pt_exact_from_env();
@@ -723,6 +725,20 @@ static int enqueue_step(const Target &x, const StepShape &sh, int step = -1) {
   return EA_OK;
 }
 
+// Sweep kind 4 writes every sweep into a second spin array; it is allocated
+// when the kind is first set and kept until the engine is destroyed.
+static int ensure_spin2(ea_engine *e) {
+  if (e->t.spin2) return EA_OK;
+  const Lattice &g = e->g();
+  return dalloc(&e->t.spin2, (size_t)g.ns * 2 * g.half);
+}
+
+// The host copy of the descriptor follows the swaps of spin and spin2 that
+// `sweeps` kind-4 sweeps make on the device.
+static void note_sweeps(ea_engine *e, const Shape &launch, long long sweeps) {
+  if (sweep_swaps(launch) && (sweeps & 1)) std::swap(e->t.spin, e->t.spin2);
+}
+
 static Target engine_target(ea_engine *e) { return {e->dt, &e->plan, 1, e->stream, e}; }
 
 // Makes ener hold the current energies by clone: sums the last sweep's
@@ -749,6 +765,7 @@ static int enqueue_sweeps(ea_engine *e, int nsweeps, bool timed) {
     Timed t(timed ? e : nullptr, EA_K_SWEEP);
     HIP_TRY(launch_sweep(e->dt, e->plan, ph, e->stream));
   }
+  note_sweeps(e, e->plan.shape, nsweeps);
   e->part_pending = true;
   e->ener_valid = false;
   return EA_OK;
@@ -863,8 +880,8 @@ static int check_shape(const ea_shape *sh) {
   if (sh->cluster_threads < 256 || sh->cluster_threads > 1024 || sh->cluster_threads % 64)
     return fail(EA_E_ARG, "cluster_threads must be a multiple of 64 in 256..1024 (got %d)",
                 sh->cluster_threads);
-  if (sh->sweep_kind < 0 || sh->sweep_kind > 3)
-    return fail(EA_E_ARG, "sweep_kind must be 0, 1, 2 or 3 (got %d)", sh->sweep_kind);
+  if (sh->sweep_kind < 0 || sh->sweep_kind > 4)
+    return fail(EA_E_ARG, "sweep_kind must be 0, 1, 2, 3 or 4 (got %d)", sh->sweep_kind);
   if (sh->sweep_kind != 0 && sh->sweep_positions != 64)
     return fail(EA_E_ARG, "sweep_kind %d takes sweep_positions 64", sh->sweep_kind);
   if (sh->sweep_xcd != 0 && sh->sweep_xcd != 1)
@@ -874,9 +891,13 @@ static int check_shape(const ea_shape *sh) {
 
 // Why a tube does not fit a sweep kind (shape_fits).
 static const char *shape_misfit(int kind) {
+  if (kind == 4)
+    return "no slab thickness T divides Lz/4 with Lz/4 >= T + 4, (T + 2) L*L/32 <= 1024 "
+           "and (2T + 6) L*L/32 <= 2048";
   return kind == 3 ? "the clones of a one-launch sweep need n = L*L*Lz/128 <= 512"
                    : "the tile (256 + L*L/16 half sites) exceeds 768";
 }
+
 
 static Shape to_shape(const ea_shape *sh) {
   Shape x;
@@ -894,6 +915,7 @@ extern "C" int ea_set_shape(ea_engine *e, const ea_shape *shape) {
   RC_TRY(check_shape(shape));
   if (!shape_fits(e->g(), to_shape(shape)))
     return fail(EA_E_ARG, "sweep_kind %d: %s", shape->sweep_kind, shape_misfit(shape->sweep_kind));
+  if (sweep_swaps(to_shape(shape))) RC_TRY(ensure_spin2(e));
   if (e->part_pending) {
     // the resolve launch exits unless a call is in flight (a previous
     // ea_run with n_it = 0 leaves none)
@@ -974,7 +996,9 @@ static int load_schedule(ea_engine *e, const ea_schedule *sch) {
                      sch->cluster ? (size_t)sch->n_it * sch->mesfr * 4 : 0);
 }
 
-static void finish_schedule(ea_engine *e, const ea_schedule *sch, uint64_t *rng_offset_out) {
+static void finish_schedule(ea_engine *e, const ea_schedule *sch, const Shape &launch,
+                            uint64_t *rng_offset_out) {
+  note_sweeps(e, launch, (long long)sch->n_it * sch->mesfr * sch->houfr * sch->ptfr);
   if (sch->n_it > 0) {
     e->part_pending = false;
     e->ener_valid = true;
@@ -998,7 +1022,7 @@ extern "C" int ea_run(ea_engine *e, const ea_schedule *sch, uint64_t *rng_offset
   // the launches carry their step index; the device step counter (read by
   // replayed batch steps only) is staged afresh by every call
   for (int it = 0; it < sch->n_it; ++it) RC_TRY(enqueue_step(x, sh, it));
-  finish_schedule(e, sch, rng_offset_out);
+  finish_schedule(e, sch, e->plan.shape, rng_offset_out);
   return EA_OK;
 }
 
@@ -1091,6 +1115,8 @@ extern "C" int ea_batch_set_shape(ea_batch *b, const ea_shape *shape) {
     if (!shape_fits(b->eng[i]->g(), to_shape(shape)))
       return fail(EA_E_ARG, "sweep_kind %d: tube %zu: %s", shape->sweep_kind, i,
                   shape_misfit(shape->sweep_kind));
+  if (sweep_swaps(to_shape(shape)))
+    for (ea_engine *e : b->eng) RC_TRY(ensure_spin2(e));
   b->launch = to_shape(shape);
   return EA_OK;
 }
@@ -1149,6 +1175,6 @@ extern "C" int ea_batch_run(ea_batch *b, const ea_schedule *sch, uint64_t *rng_o
   for (int k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(b->exec, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   for (int i = 0; i < n; ++i)
-    finish_schedule(b->eng[i], &sch[i], rng_offsets_out ? rng_offsets_out + i : nullptr);
+    finish_schedule(b->eng[i], &sch[i], b->launch, rng_offsets_out ? rng_offsets_out + i : nullptr);
   return EA_OK;
 }

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <climits>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -30,6 +31,7 @@ typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 // pointers, and loads through them would compile to flat loads, whose waits
 // also drain LDS traffic and serialise the samples' loads.
 typedef __attribute__((address_space(1))) uint64_t gword;
+typedef __attribute__((address_space(1))) uint8_t gbyte;
 __device__ __forceinline__ gword *gmem(uint64_t *p) { return (gword *)p; }
 __device__ __forceinline__ const gword *gmem(const uint64_t *p) { return (const gword *)p; }
 // The same for any element type.  Stores through a flat pointer also count
@@ -324,8 +326,7 @@ __device__ __forceinline__ void load_lane(const gword *S, const gword *Jb, uint3
 // Neighbours brought to the centre's copy layout (load_neighbours), as
 // selects rather than branches so that no wait sits between two samples'
 // loads.
-__device__ __forceinline__ void fix_lane(const LaneSite &ls, int parity, SiteIn &x) {
-  const int w = ls.wrap;
+__device__ __forceinline__ void fix_flags(int w, bool top, bool bottom, int parity, SiteIn &x) {
   x.sc[0] = (w & 1) ? wrap_xp(x.sc[0]) : x.sc[0];
   x.sc[1] = (w & 2) ? wrap_yp(x.sc[1]) : x.sc[1];
   x.sc[2] = (w & 4) ? wrap_zp(x.sc[2]) : x.sc[2];
@@ -333,8 +334,11 @@ __device__ __forceinline__ void fix_lane(const LaneSite &ls, int parity, SiteIn 
   x.sc[4] = (w & 16) ? wrap_ym(x.sc[4]) : x.sc[4];
   x.sc[5] = (w & 32) ? wrap_zm(x.sc[5]) : x.sc[5];
   const uint64_t aux = parity == 0 ? x.sc[1] : x.sc[4];
-  x.sc[2] = ls.top ? (x.sc[2] & ~kMaskD) | (aux & kMaskD) : x.sc[2];
-  x.sc[5] = ls.bottom ? (x.sc[5] & ~kMaskU) | (aux & kMaskU) : x.sc[5];
+  x.sc[2] = top ? (x.sc[2] & ~kMaskD) | (aux & kMaskD) : x.sc[2];
+  x.sc[5] = bottom ? (x.sc[5] & ~kMaskU) | (aux & kMaskU) : x.sc[5];
+}
+__device__ __forceinline__ void fix_lane(const LaneSite &ls, int parity, SiteIn &x) {
+  fix_flags(ls.wrap, ls.top, ls.bottom, parity, x);
 }
 
 // v_bitop3_b32 (gfx950): bit i of the result is bit (a_i << 2 | b_i << 1 |
@@ -970,6 +974,398 @@ __global__ void __launch_bounds__(kFullThreads) k_sweep_full(const Tube *__restr
   }
   if (act) *chunk = *reinterpret_cast<const u64x2 *>(sp + 2 * tx);
   if (!own && next) make_words();
+}
+
+
+// ------------------------------------------------------------ sweep kind 4
+
+typedef __attribute__((address_space(3))) uint8_t lbyte;
+
+// LDS byte offset, within a sample's buffer, of the word of parity `par` at
+// half index hh of a slab whose first plane is z0: odd words of planes
+// z0-2 .. z0+T+1 first, then even words of planes z0-1 .. z0+T, each range
+// taken mod MLz (so a slab at either end of the tube reads its halo planes
+// from the other end, as the periodic z neighbours and the open-boundary
+// substitutes of main.cu:1440-1472 need).
+__device__ __forceinline__ uint32_t slab_offset(const Lattice &g, int T, int z0, int par,
+                                                int hh) {
+  const int pl = hh / g.hs, rr = hh - pl * g.hs;
+  int lp = pl - z0 + (par ? 2 : 1);
+  lp += lp < 0 ? g.MLz : 0;
+  lp -= lp >= g.MLz ? g.MLz : 0;
+  return 8u * (uint32_t)((par ? 0 : (T + 4) * g.hs) + lp * g.hs + rr);
+}
+
+// One position a thread updates in every sample, packed into two words:
+// `own` holds the LDS byte offset of its word in a sample buffer (bits
+// 0-15), the directions whose neighbour wrapped around the sub-lattice (bits
+// 16-21) and the open-boundary top and bottom flags (bits 22, 23); `dxy`
+// holds, as signed bytes, the in-plane word offsets of the +x, -x, +y and -y
+// neighbours from the position's own half index within its plane.  The z
+// neighbours sit at the same in-plane index one plane up or down, and the
+// other parity's copy of the position's plane lies at a fixed distance from
+// its own word, so every neighbour's offset is own + a per-phase constant
+// (+ 8 dxy for x and y).
+struct SlabSite {
+  uint32_t own, dxy;
+  int h;
+};
+
+__device__ __forceinline__ SlabSite slab_site(const Lattice &g, int T, int z0, int p, int gz,
+                                              int rem) {
+  SlabSite ss;
+  ss.h = gz * g.hs + rem;
+  const Nbrs nb = neighbours(g, p, ss.h);
+  const uint32_t own = slab_offset(g, T, z0, p, ss.h);
+  const bool top = g.bc == EA_OBC && nb.gz == g.MLz - 1;
+  const bool bottom = g.bc == EA_OBC && nb.gz == 0;
+  ss.own = own | (uint32_t)nb.wrap << 16 | (uint32_t)top << 22 | (uint32_t)bottom << 23;
+  const int dirs[4] = {0, 3, 1, 4};  // +x, -x, +y, -y
+  ss.dxy = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int hn = nb.h[dirs[k]];
+    const int d = (hn - (hn / g.hs) * g.hs) - rem;
+    ss.dxy |= (uint32_t)(d & 0xFF) << (8 * k);
+  }
+  return ss;
+}
+
+// The seven words of a packed position from a sample buffer at LDS byte
+// offset buf: cxy, czp, czm are the phase's distances from the own word to
+// the other parity's copy of the same plane and of the planes above and
+// below.
+template <typename LW>
+__device__ __forceinline__ void slab_words(LW lword, uint32_t buf, const SlabSite &ss,
+                                           int cxy, int czp, int czm, SiteIn &x) {
+  const uint32_t own = (ss.own & 0xFFFFu) + buf;
+  x.sv = lword(own);
+  const uint32_t pl = own + (uint32_t)cxy;
+  const int dx = (int)(ss.dxy << 24) >> 24, dxm = (int)(ss.dxy << 16) >> 24;
+  const int dy = (int)(ss.dxy << 8) >> 24, dym = (int)ss.dxy >> 24;
+  x.sc[0] = lword(pl + 8 * dx);
+  x.sc[3] = lword(pl + 8 * dxm);
+  x.sc[1] = lword(pl + 8 * dy);
+  x.sc[4] = lword(pl + 8 * dym);
+  x.sc[2] = lword(own + (uint32_t)czp);
+  x.sc[5] = lword(own + (uint32_t)czm);
+}
+
+// The wrap fix-ups of fix_flags without selects: each x or y permutation
+// is (a & m) | (b & ~m) of two shifts of the word, and a lane that does not
+// wrap shifts by 0, which leaves the word as it is.  Both forms, with the
+// shift amounts of each direction (1/3 for x, 4/12 for y), equal the
+// wrap_* functions bit for bit.  The rare z wraps (first and last MSC plane)
+// keep wrap_zp/wrap_zm behind a branch that almost every wave skips.
+constexpr uint32_t kSelAB = (kT3A & kT3C) | (kT3B & ~kT3C & 0xFF);  // (a & c) | (b & ~c)
+__device__ __forceinline__ uint64_t wrap_up(uint64_t o, uint32_t f, uint32_t a, uint32_t b,
+                                            uint64_t m) {
+  return op3<kSelAB>(o << (f * a), o >> (f * b), m);
+}
+__device__ __forceinline__ uint64_t wrap_down(uint64_t o, uint32_t f, uint32_t a, uint32_t b,
+                                              uint64_t m) {
+  return op3<kSelAB>(o >> (f * a), o << (f * b), m);
+}
+__device__ __forceinline__ void fix_flags_sel(uint32_t w, bool top, bool bottom, int parity,
+                                              SiteIn &x) {
+  x.sc[0] = wrap_up(x.sc[0], w & 1, 3, 1, 0x8888888888888888ULL);
+  x.sc[3] = wrap_down(x.sc[3], (w >> 3) & 1, 3, 1, 0x1111111111111111ULL);
+  x.sc[1] = wrap_up(x.sc[1], (w >> 1) & 1, 12, 4, 0xF000F000F000F000ULL);
+  x.sc[4] = wrap_down(x.sc[4], (w >> 4) & 1, 12, 4, 0x000F000F000F000FULL);
+  if (w & 4) x.sc[2] = wrap_zp(x.sc[2]);
+  if (w & 32) x.sc[5] = wrap_zm(x.sc[5]);
+  const uint64_t aux = parity == 0 ? x.sc[1] : x.sc[4];
+  x.sc[2] = top ? (x.sc[2] & ~kMaskD) | (aux & kMaskD) : x.sc[2];
+  x.sc[5] = bottom ? (x.sc[5] & ~kMaskU) | (aux & kMaskU) : x.sc[5];
+}
+
+// A copy the compiler cannot see through: per-lane byte offsets made
+// inside a loop stay 32-bit there (a uniform base plus a 32-bit offset is
+// one load with a scalar base), instead of being hoisted as 64-bit
+// addresses that hold twice the registers for the whole loop.
+__device__ __forceinline__ uint32_t opaque(uint32_t v) {
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "v"(v));
+  return v;
+}
+
+// One sweep, kind 4: both half-sweeps (ph.half, even, and ph.half + 1) in
+// one launch, for tubes whose clones do not fit one block's LDS.  Block
+// (slab, replica r, beta), beta fastest: T = a.sweep_planes MSC planes
+// [z0, z0 + T) of the clone at beta of replica r, every sample in turn.
+// Per sample the block stages in LDS the odd words of planes z0-2 .. z0+T+1
+// and the even words of planes z0-1 .. z0+T (from a.spin), updates the even
+// words of planes z0-1 .. z0+T in LDS -- the two outer planes belong to the
+// neighbouring slabs, which make the same new words from the same inputs --
+// then the odd words of its T planes from the new even ones, and writes the
+// new words of its T planes to a.spin2: no block reads a word that another
+// has already rewritten, and every word of a.spin2 is written once.  The
+// next sample's words load while this one is updated, into the other of
+// two LDS buffers.  The thresholds of the block's (2T + 2) hs positions are
+// made once, from the two halves' counter blocks, and serve every sample
+// (the reference's one set of random words per half-sweep for all samples,
+// main.cu:299,308).  One energy partial per (sample, replica, beta, slab)
+// when ph.energy, from the odd half (every bond has exactly one odd end).
+template <bool XCD>
+__global__ void __launch_bounds__(kSlabThreads, 4) k_sweep_slab(const Tube *__restrict__ tubes,
+                                                            const uint64_t *__restrict__ map,
+                                                            Phase ph) {
+  __shared__ __align__(16) uint32_t lut[EA_LUT_BYTES / 4];
+  __shared__ uint32_t crow[EA_MAX_SAMPLES];  // clone row of (s, r) at beta, in words
+  __shared__ uint32_t ctr[2][5];
+  __shared__ int esum[2];
+  extern __shared__ __align__(16) uint64_t dyn[];  // buffers [2][kSlabBuf], thresholds
+  const Where wh = locate_block(tubes, map, ph.step,
+                                XCD ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x);
+  const Tube &a = *wh.t;
+  if (wh.step >= a.nsteps) return;
+  const Lattice g = a.g;
+  const int T = a.sweep_planes, tx = threadIdx.x;
+  const int rest = wh.local / g.nt, beta = wh.local - rest * g.nt;
+  const int r = rest & (g.nr - 1), slab = rest >> g.nr_shift, z0 = slab * T;
+  const int hidx = wh.step * a.halves + ph.half;
+  const int ne = (T + 2) * g.hs, no = T * g.hs;  // positions of each parity
+  const int nch = sweep_slab_words(g, T) / 2;     // 16-byte chunks per sample
+
+  // tables: every load first, into registers
+  constexpr int kLutChunks = (EA_LUT_BYTES / 16 + kSlabThreads - 1) / kSlabThreads;
+  u32x4 lv[kLutChunks];
+  {
+    const auto src =
+        reinterpret_cast<const global_t<u32x4> *>(gv(a.lut + (size_t)beta * EA_LUT_BYTES));
+#pragma unroll
+    for (int i = 0; i < kLutChunks; ++i) lv[i] = src[min(tx + i * kSlabThreads, EA_LUT_BYTES / 16 - 1)];
+  }
+  int clone = 0;
+  if (tx < g.ns) clone = gv(a.wctb)[((size_t)tx * g.nr + r) * g.nt + beta];
+  uint32_t cw[4] = {0, 0, 0, 0};
+  if (tx < 2) {
+    const auto c = gv(a.ctr) + ((size_t)hidx + tx) * g.nr + r;
+    cw[0] = c->x;
+    cw[1] = c->y;
+    cw[2] = c->z;
+    cw[3] = c->w;
+  }
+  // the thread's chunks of a sample buffer: LDS byte offset and word offset
+  // within a sample's spins (parity array, then the clone row's plane)
+  uint32_t cl[kSlabChunks], cg[kSlabChunks];
+#pragma unroll
+  for (int k = 0; k < kSlabChunks; ++k) {
+    const int c = min(tx + k * kSlabThreads, nch - 1), w = 2 * c;
+    const int odd = w < (T + 4) * g.hs;
+    const int w2 = odd ? w : w - (T + 4) * g.hs;
+    const int lp = w2 / g.hs, off = w2 - lp * g.hs;
+    int gz = z0 + lp - (odd ? 2 : 1);
+    gz += gz < 0 ? g.MLz : 0;
+    gz -= gz >= g.MLz ? g.MLz : 0;
+    cl[k] = 8u * (uint32_t)w;
+    cg[k] = (uint32_t)(odd ? g.half : 0) + (uint32_t)(gz * g.hs + off);
+  }
+  {
+    u32x4 *dst = reinterpret_cast<u32x4 *>(lut);
+#pragma unroll
+    for (int i = 0; i < kLutChunks; ++i)
+      if (tx + i * kSlabThreads < EA_LUT_BYTES / 16) dst[tx + i * kSlabThreads] = lv[i];
+  }
+  if (tx < g.ns) crow[tx] = (uint32_t)spin_row(g, r, clone);
+  if (tx < 2) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ctr[tx][k] = cw[k];
+    ctr[tx][4] = a.keys[r];
+    esum[tx] = 0;
+  }
+  lds_barrier();
+
+  const size_t sstride = 2 * (size_t)g.half;
+  const gword *src = gmem(a.spin);
+  gword *dst = gmem(a.spin2);
+  lbyte *const lds = (lbyte *)(dyn);
+  typedef __attribute__((address_space(3))) uint64_t lu64;
+  auto lword = [&](uint32_t off) -> lu64 & { return *(lu64 *)(lds + off); };
+  u64x2 tv[kSlabChunks];
+  auto fetch = [&](int s) {
+    const gbyte *S2 =
+        (const gbyte *)(src + (size_t)s * sstride + __builtin_amdgcn_readfirstlane(crow[s]));
+#pragma unroll
+    for (int k = 0; k < kSlabChunks; ++k)
+      tv[k] = *(const global_t<u64x2> *)(S2 + opaque(8u * cg[k]));
+  };
+  auto stage = [&](uint32_t buf) {
+#pragma unroll
+    for (int k = 0; k < kSlabChunks; ++k)
+      *(__attribute__((address_space(3))) u64x2 *)(lds + buf + cl[k]) = tv[k];
+  };
+  fetch(0);  // in flight while the thresholds are made
+  // thresholds of positions pos = tx, tx + kSlabThreads, ...: even ones
+  // (pos < ne) with the even half's counter block, odd ones with the odd's
+  const uint32_t thr0 = 2 * kSlabBuf;
+  for (int pos = tx; pos < ne + no; pos += kSlabThreads) {
+    const int odd = pos >= ne, q = odd ? pos - ne : pos;
+    const int lp = q / g.hs, rem = q - lp * g.hs;
+    int gz = z0 + lp - (odd ? 0 : 1);
+    gz += gz < 0 ? g.MLz : 0;
+    gz -= gz >= g.MLz ? g.MLz : 0;
+    const uint32_t hh = (uint32_t)(gz * g.hs + rem);
+    const uint32_t tid = ((uint32_t)beta * (uint32_t)g.n + hh) * (uint32_t)g.nr + r;
+    const uint32_t(&c)[5] = ctr[odd];
+    uint32_t w[6];
+#pragma unroll 1
+    for (int k = 0; k < 6; ++k) w[k] = rnd_word(c[0] + k, c[1], c[2] + tid, c[3], tid, c[4], lut);
+    uint64_t i1, i2;
+    thresholds(w, i1, i2);
+    *(__attribute__((address_space(3))) u64x2 *)(lds + thr0 + 16u * pos) = u64x2{i1, i2};
+  }
+  // (made after the thresholds, so that fewer values are live there)
+  // the thread's positions: even ones tx, tx + kSlabThreads of planes
+  // z0-1 .. z0+T, odd ones of planes z0 .. z0+T-1; a thread past the end
+  // takes position 0's addressing and stores nothing
+  SlabSite es[kSlabPos], os[kSlabPos];
+  bool ev[kSlabPos], ov[kSlabPos], est[kSlabPos];
+#pragma unroll
+  for (int i = 0; i < kSlabPos; ++i) {
+    const int pe = tx + i * kSlabThreads, qe = pe < ne ? pe : 0;
+    ev[i] = pe < ne;
+    const int le = qe / g.hs, re = qe - le * g.hs;
+    int gz = z0 - 1 + le;
+    gz += gz < 0 ? g.MLz : 0;
+    gz -= gz >= g.MLz ? g.MLz : 0;
+    es[i] = slab_site(g, T, z0, 0, gz, re);
+    est[i] = ev[i] && le >= 1 && le <= T;
+    const int po = tx + i * kSlabThreads, qo = po < no ? po : 0;
+    ov[i] = po < no;
+    const int lo = qo / g.hs;
+    os[i] = slab_site(g, T, z0, 1, z0 + lo, qo - lo * g.hs);
+  }
+  stage(0);
+  lds_barrier();
+
+  const uint32_t jpb = 16 * (uint32_t)g.n, jodd = 3 * jpb;  // direction pair, odd parity (bytes)
+  // distances (bytes) from an even word of local plane le (LDS offset
+  // 8 (E0 + le hs + rem)) to the odd words of the same plane and of the planes
+  // above and below (odd local planes le + 1, le + 2, le); an odd word's to
+  // the even ones are their negatives
+  const int E0 = (T + 4) * g.hs;
+  const int exy = 8 * (g.hs - E0), ezp = 8 * (2 * g.hs - E0), ezm = -8 * E0;
+  const uint32_t pstride = (uint32_t)(g.nr * g.nt * a.nbx);
+  const uint32_t pbase = (uint32_t)((r * g.nt + beta) * a.nbx + slab);
+  const int last = g.ns - 1;
+  // the passes in which this wave has a valid position (uniform per wave)
+  const int w0 = __builtin_amdgcn_readfirstlane(tx) & ~63;
+  bool wev[kSlabPos], wov[kSlabPos];
+#pragma unroll
+  for (int i = 0; i < kSlabPos; ++i) {
+    wev[i] = w0 + i * kSlabThreads < ne;
+    wov[i] = w0 + i * kSlabThreads < no;
+  }
+  // couplings of sample s (bytes from its start); the even half's of the
+  // next sample are loaded a sample ahead (a wave with no valid position in
+  // a pass skips its loads and its update)
+  auto jbase = [&](int s) { return (const gbyte *)(gmem(a.J) + (size_t)s * 12 * g.n); };
+  u64x2 Je[kSlabPos][3];
+  auto load_even = [&](int s) {
+    const gbyte *Jb = jbase(s);
+#pragma unroll
+    for (int i = 0; i < kSlabPos; ++i) {
+      const uint32_t je = opaque(16u * (uint32_t)es[i].h);
+      if (wev[i]) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Je[i][k] = *(const global_t<u64x2> *)(Jb + (je + k * jpb));
+      }
+    }
+  };
+  load_even(0);
+  // one sample, its words in the buffer at LDS byte offset BUF
+  auto sample = [&](int s, auto bufc) {
+    constexpr uint32_t BUF = decltype(bufc)::value;
+    fetch(min(s + 1, last));  // the next sample's words (after the last: a repeat)
+    // uniform bases and 32-bit byte offsets (within a sample's spins or
+    // couplings), so that the loads and stores take a scalar base
+    gbyte *Db = (gbyte *)(dst + (size_t)s * sstride + __builtin_amdgcn_readfirstlane(crow[s]));
+    // the odd half's couplings arrive during the even half; the even half's
+    // came in during the previous sample's odd half (load_even)
+    u64x2 Jd[kSlabPos][3];
+    {
+      const gbyte *Jb = jbase(s);
+#pragma unroll
+      for (int i = 0; i < kSlabPos; ++i) {
+        const uint32_t jd = opaque(16u * (uint32_t)os[i].h + jodd);
+        if (wov[i]) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) Jd[i][k] = *(const global_t<u64x2> *)(Jb + (jd + k * jpb));
+        }
+      }
+    }
+    // even half: planes z0-1 .. z0+T, in LDS; planes z0 .. z0+T-1 also to spin2
+#pragma unroll
+    for (int i = 0; i < kSlabPos; ++i) {
+      if (!wev[i]) break;  // the passes' valid waves only shrink
+      // one pass at a time: both in flight together would need more
+      // registers than four waves per SIMD leave
+      __builtin_amdgcn_sched_barrier(0);
+      const SlabSite &ss = es[i];
+      SiteIn x;
+#pragma unroll
+      for (int d = 0; d < 6; ++d) x.J[d] = Je[i][d >> 1][d & 1];
+      slab_words(lword, BUF, ss, exy, ezp, ezm, x);
+      const uint32_t ti = ev[i] ? (uint32_t)(tx + i * kSlabThreads) : 0u;
+      const u64x2 id = *(__attribute__((address_space(3))) u64x2 *)(lds + thr0 + 16u * ti);
+      const uint32_t fl = opaque(ss.own) >> 16;  // derived per sample, not held
+      fix_flags_sel(fl & 63, (fl >> 6) & 1, (fl >> 7) & 1, 0, x);
+      int e = 0;
+      const uint64_t nw = x.sv ^ site_flip(x, id[0], id[1], false, e);
+      if (ev[i]) lword(BUF + (ss.own & 0xFFFFu)) = nw;
+      if (est[i]) *(global_t<uint64_t> *)(Db + opaque(8u * (uint32_t)ss.h)) = nw;
+    }
+    lds_barrier();
+    load_even(min(s + 1, last));  // the next sample's even couplings
+    // odd half: planes z0 .. z0+T-1, from the new even words
+    int e = 0;
+#pragma unroll
+    for (int i = 0; i < kSlabPos; ++i) {
+      if (!wov[i]) break;
+      __builtin_amdgcn_sched_barrier(0);
+      const SlabSite &ss = os[i];
+      const bool vo = ov[i];
+      SiteIn x;
+#pragma unroll
+      for (int d = 0; d < 6; ++d) x.J[d] = Jd[i][d >> 1][d & 1];
+      slab_words(lword, BUF, ss, -exy, -ezm, -ezp, x);
+      const uint32_t ti = (uint32_t)ne + (vo ? (uint32_t)(tx + i * kSlabThreads) : 0u);
+      const u64x2 id = *(__attribute__((address_space(3))) u64x2 *)(lds + thr0 + 16u * ti);
+      const uint32_t fl = opaque(ss.own) >> 16;
+      fix_flags_sel(fl & 63, (fl >> 6) & 1, (fl >> 7) & 1, 1, x);
+      int ei = 0;
+      const uint64_t nw = x.sv ^ site_flip(x, id[0], id[1], ph.energy, ei);
+      if (vo) {
+        *(global_t<uint64_t> *)(Db + opaque(8u * (uint32_t)(g.half + ss.h))) = nw;
+        e += ei;
+      }
+    }
+    if (ph.energy) {
+      e = group_scan<64>(e);
+      if ((tx & 63) == 63) atomicAdd(&esum[s & 1], e);
+    }
+    stage(BUF ^ kSlabBuf);  // the next sample's words, into the other buffer
+    lds_barrier();
+    if (ph.energy && tx == 0) {
+      gv(a.part)[(uint32_t)s * pstride + pbase] = esum[s & 1];
+      esum[s & 1] = 0;
+    }
+  };
+  for (int s = 0; s < g.ns; s += 2) {
+    sample(s, std::integral_constant<uint32_t, 0>());
+    if (s + 1 < g.ns) sample(s + 1, std::integral_constant<uint32_t, kSlabBuf>());
+  }
+}
+
+// After a kind-4 sweep: the tubes that ran it swap their two spin arrays.
+__global__ void k_swap_spins(Tube *tubes, int ntubes, int step) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntubes) return;
+  Tube &t = tubes[i];
+  if ((step >= 0 ? step : *t.step) >= t.nsteps) return;
+  uint64_t *const a = t.spin;
+  t.spin = t.spin2;
+  t.spin2 = a;
 }
 
 // ----------------------------------------------------------------- energy
@@ -2726,11 +3122,16 @@ inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) /
 
 int sweep_partials_x(const Lattice &g, const Shape &shape) {
   if (shape.sweep_kind == 3) return 1;
+  if (shape.sweep_kind == 4) {
+    const int T = sweep_slab_planes(g);
+    return T ? g.MLz / T : 1;
+  }
   return (int)blocks_for(g.n, shape.sweep_kind ? 64 : shape.sweep_positions);
 }
 
 bool shape_fits(const Lattice &g, const Shape &shape) {
   if (shape.sweep_kind == 3) return g.n <= kFullMaxN;
+  if (shape.sweep_kind == 4) return sweep_slab_planes(g) > 0;
   return shape.sweep_kind != 2 || tile_words(g) <= kTileWordsMax;
 }
 
@@ -2738,6 +3139,7 @@ static long long sweep_grid(const Tube &t, const Shape &shape) {
   const Lattice &g = t.g;
   if (shape.sweep_kind == 3)
     return blocks_for(g.ns, sweep_full_samples(g)) * g.nr * g.nt;
+  if (shape.sweep_kind == 4) return (long long)(g.MLz / t.sweep_planes) * g.nr * g.nt;
   const long long per = shape.sweep_kind ? blocks_for(g.n, kLaneThreads) : t.nbx;
   return per * g.nr * g.nt;
 }
@@ -2814,6 +3216,8 @@ Plan make_plan(const std::vector<Tube> &tubes, Shape shape) {
     add(kMapSweep, sweep_grid(t, shape));
     if (shape.sweep_kind == 2) p.tile_words = std::max(p.tile_words, tile_chunks(g));
     if (shape.sweep_kind == 3) p.lds_full = std::max(p.lds_full, sweep_full_lds(g));
+    if (shape.sweep_kind == 4)
+      p.lds_full = std::max(p.lds_full, sweep_slab_lds(g, t.sweep_planes));
     add(kMapEnergy, (long long)energy_blocks_host(g) * g.nt * g.ns);
     add(kMapChain, (long long)g.ns * g.nr);
     add(kMapOverlap, (long long)g.ns * (g.nr * (g.nr - 1) / 2) * g.nt);
@@ -2859,6 +3263,7 @@ hipError_t upload_plan(const Plan &p, DevPlan *d) {
   d->lds_apply = p.lds_apply;
   d->lds_seam = p.lds_seam;
   d->lds_full = p.lds_full;
+  d->ntubes = p.ntubes;
   d->tile_words = p.tile_words;
   d->shape = p.shape;
   d->identity = p.ntubes == 1;
@@ -2902,7 +3307,15 @@ hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream
   const dim3 grid((unsigned)p.cnt[kMapSweep]);
   const uint64_t *m = p.map(kMapSweep);
   const Shape &sh = p.shape;
-  if (sh.sweep_kind == 3) {
+  if (sh.sweep_kind == 4) {
+    auto k = sh.sweep_xcd ? k_sweep_slab<true> : k_sweep_slab<false>;
+    allow_lds(reinterpret_cast<const void *>(k), p.lds_full);
+    hipLaunchKernelGGL(k, grid, dim3(kSlabThreads), p.lds_full, s, tubes, m, ph);
+    // the descriptors are not written by any kernel but this one, which runs
+    // after the sweep has finished reading them
+    hipLaunchKernelGGL(k_swap_spins, dim3(blocks_for(p.ntubes, 64)), dim3(64), 0, s,
+                       const_cast<Tube *>(tubes), p.ntubes, ph.step);
+  } else if (sh.sweep_kind == 3) {
     auto k = sh.sweep_xcd ? k_sweep_full<true> : k_sweep_full<false>;
     allow_lds(reinterpret_cast<const void *>(k), p.lds_full);
     hipLaunchKernelGGL(k, grid, dim3(kFullThreads), p.lds_full, s, tubes, m, ph);

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <vector>
@@ -115,6 +116,11 @@ struct Tube {
   // [2 buffers][nr][nt][2 parities][n][2], made a slice per sample group by
   // the sweep before (NULL when n > kFullMaxN)
   uint64_t *thr;
+  // sweep kind 4: the array the new words of a sweep go to, swapped with
+  // `spin` after every such sweep (NULL until the kind is first set), and the
+  // MSC planes of one sweep slab (sweep_slab_planes)
+  uint64_t *spin2;
+  int sweep_planes;
   int planes;               // slab Houdayer planes; 0: whole tube in one block
   int hou_cap;              // run nodes of the whole-tube labelling (hou_run_cap)
   int nbx;                  // sweep blocks per beta
@@ -157,7 +163,8 @@ enum MapKind {
 // and all samples (256-thread blocks, one energy partial per wave);
 // kind 2 is kind 1 with the neighbour words staged in LDS tiles; kind 3
 // makes a whole sweep (both halves) in one launch, each block holding whole
-// clones in LDS (tubes with n <= kFullMaxN);
+// clones in LDS (tubes with n <= kFullMaxN); kind 4 makes a whole sweep in
+// one launch over z-slabs of the clones, into a second spin array;
 // sweep_xcd = 1 hands each XCD a contiguous range of sweep blocks.  cluster_threads: threads of
 // a whole-tube Houdayer workgroup (256, 512 or 1024).
 struct Shape {
@@ -193,11 +200,47 @@ inline size_t sweep_full_thr_words(const Lattice &g) {
   return g.n <= kFullMaxN ? (size_t)8 * g.nr * g.nt * g.n : 0;
 }
 
+// Kind 4 (one launch per sweep, for tubes that kind 3 does not take): a
+// block of kSlabThreads threads owns a z-slab of T = sweep_slab_planes(g) MSC
+// planes of one clone and takes the samples in turn, staging in LDS the odd
+// words of planes z0-2 .. z0+T+1 and the even words of planes z0-1 .. z0+T
+// (two buffers, one sample ahead), and the thresholds of the 2T+2 planes'
+// positions it updates; it reads Tube::spin and writes Tube::spin2.
+// T divides MLz, leaves at most kSlabPos positions of each parity per thread,
+// keeps the halo planes distinct (MLz >= T + 4) and the LDS within
+// kSlabLdsMax; 0 when no T does (the kind does not fit the tube).
+constexpr int kSlabThreads = 512;
+constexpr int kSlabPos = 2;
+constexpr int kSlabChunks = 2;        // 16-byte chunks per thread and sample
+constexpr uint32_t kSlabBuf = 16384;  // LDS bytes of one sample's buffer
+constexpr size_t kSlabLdsMax = 69 * 1024;  // dynamic LDS: two blocks per CU
+__host__ __device__ inline int sweep_slab_words(const Lattice &g, int T) {
+  return (2 * T + 6) * g.hs;  // one sample's buffer: odd then even planes
+}
+__host__ __device__ inline size_t sweep_slab_lds(const Lattice &g, int T) {
+  // two sample buffers, then (id1, id2) of the (2T + 2) hs positions
+  return 2 * (size_t)kSlabBuf + 16 * (size_t)(2 * T + 2) * g.hs;
+}
+inline int sweep_slab_planes(const Lattice &g) {
+  if (16 * g.half >= (1LL << 32)) return 0;  // 32-bit byte offsets within a sample
+  // one energy partial per slab: within the slots every chain has
+  // (energy_partials)
+  const long long slots = std::max<long long>((g.n + 31) / 32, 4LL * ((g.n * g.nr + 255) / 256));
+  for (int T = g.MLz; T >= 2; --T) {
+    if (g.MLz % T || g.MLz < T + 4 || g.MLz / T > slots) continue;
+    if ((T + 2) * g.hs > kSlabPos * kSlabThreads) continue;
+    if (8 * (size_t)sweep_slab_words(g, T) > kSlabBuf) continue;
+    if (sweep_slab_lds(g, T) > kSlabLdsMax) continue;
+    return T;
+  }
+  return 0;
+}
+
 struct Plan {
   std::vector<uint64_t> map[kMapKinds];
   Shape shape;
   size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0, lds_seam = 0;
-  size_t lds_full = 0;  // dynamic LDS of a kind-3 sweep (the largest tube's)
+  size_t lds_full = 0;  // dynamic LDS of a kind-3 or kind-4 sweep (the largest tube's)
   int tile_words = 0;  // 16-byte tile chunks per thread of a tiled sweep (the widest tube's)
   int ntubes = 0;
 };
@@ -212,6 +255,7 @@ struct DevPlan {
   size_t lds16 = 0, lds32 = 0, lds64 = 0, lds_slab = 0, lds_apply = 0, lds_seam = 0;
   size_t lds_full = 0;
   int tile_words = 0;  // 16-byte tile chunks per thread of a tiled sweep (the widest tube's)
+  int ntubes = 0;
   Shape shape;
   bool identity = false;
   const uint64_t *map(int k) const { return identity ? nullptr : buf + off[k]; }
@@ -237,7 +281,10 @@ void free_plan(DevPlan *d);
 // One half-sweep (ph.half), or with kind 3 (sweep_is_whole) the whole sweep
 // of halves ph.half (even) and ph.half + 1.
 hipError_t launch_sweep(const Tube *tubes, const DevPlan &p, Phase ph, hipStream_t s);
-inline bool sweep_is_whole(const Shape &sh) { return sh.sweep_kind == 3; }
+inline bool sweep_is_whole(const Shape &sh) { return sh.sweep_kind >= 3; }
+// Kind 4 writes the new words to Tube::spin2; after each of its launches the
+// descriptors of the tubes that ran swap spin and spin2.
+inline bool sweep_swaps(const Shape &sh) { return sh.sweep_kind == 4; }
 hipError_t launch_energy(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_overlap(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);
 hipError_t launch_resolve(const Tube *tubes, const DevPlan &p, int step, hipStream_t s);

@@ -585,7 +585,8 @@ class Simulation:
 # sweep_kind, sweep_xcd): the sweep options, and the Houdayer workgroup
 # sizes.
 SWEEP_OPTIONS = ((64, 0, 0), (32, 0, 0), (64, 0, 1), (32, 0, 1), (64, 1, 0),
-                 (64, 1, 1), (64, 2, 0), (64, 2, 1), (64, 3, 0), (64, 3, 1))
+                 (64, 1, 1), (64, 2, 0), (64, 2, 1), (64, 3, 0), (64, 3, 1),
+                 (64, 4, 0), (64, 4, 1))
 CLUSTER_THREADS = (1024, 512, 256)
 DEFAULT_SHAPE = (64, 1024, 0, 0)
 SHAPES = tuple((p, c, k, x) for (p, k, x) in SWEEP_OPTIONS

@@ -120,11 +120,13 @@ def test_every_launch_shape_gives_the_same_state():
     want = [state(p) for p in ref]
     for shp in eatube.SHAPES:
         say("batch shape", shp)
-        twins = [make(s, 30 + k) for k, s in enumerate(specs)]
+        # an 8x16 tube has too few planes for kind 4's slabs and their halos
+        tubes = [k for k in range(len(specs)) if shp[2] != 4 or k > 0]
+        twins = [make(specs[k], 30 + k) for k in tubes]
         batch = eatube.Batch([t.engine for t in twins])
         batch.set_shape(*shp)
         batch.run([t.schedule(2, cluster=True) for t in twins])
-        for k, t in enumerate(twins):
+        for k, t in zip(tubes, twins):
             same(want[k], state(t), "batch shape %s tube %d" % (shp, k))
         batch.close()
     for shp in eatube.SHAPES:
@@ -180,7 +182,8 @@ def test_launch_shapes_with_other_replica_counts(nr):
         return state(x)
 
     want = sim(None)
-    for shp in ((32, 512, 0, 1), (64, 1024, 1, 0), (64, 256, 2, 1), (64, 512, 3, 0)):
+    for shp in ((32, 512, 0, 1), (64, 1024, 1, 0), (64, 256, 2, 1), (64, 512, 3, 0),
+                (64, 512, 4, 1)):
         say("nr", nr, "shape", shp)
         same(want, sim(shp), "nr %d shape %s" % (nr, shp))
 

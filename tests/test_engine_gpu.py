@@ -504,6 +504,55 @@ def test_one_launch_sweeps_match_oracle(L, Lz, nt, ns, bc, ptfr):
     assert np.array_equal(e.engine.energy().ravel(), o.field("ener")), "kind 3 energies"
 
 
+@pytest.mark.parametrize("L,Lz,nt,ns,bc,ptfr", [(16, 64, 6, 31, 1, 1), (16, 64, 4, 5, 0, 2),
+                                                  (24, 24, 4, 3, 1, 2), (48, 96, 4, 7, 0, 1),
+                                                  (32, 64, 3, 4, 1, 1)])
+def test_slab_sweeps_match_oracle(L, Lz, nt, ns, bc, ptfr):
+    """Sweep kind 4 (both half-sweeps in one launch over z-slabs of the
+    clones, into a second spin array swapped after every sweep) against the
+    oracle, through ea_run with cluster moves and through ea_sweep with an
+    odd and then an even number of sweeps: slabs of 8 planes (two of them,
+    each the other's halo across the periodic ends), of 2 planes (three,
+    open boundaries), three slabs of 8 planes at L=48 and two at L=32; odd
+    sample counts end the sample loop in the first LDS buffer."""
+    o, e = pair(L, Lz, 4, nt, ns, bc=bc, seeds=(81, 82, 83, 84 + bc))
+    e.engine.set_shape(64, 512, 4, 1)
+    kw = dict(ptfr=ptfr, cluster=True)
+    e.steps(3, it0=0, frec_trw=1, **kw)
+    o.run(3, it0=0, frec_trw=1, rows=3, **kw)
+    assert_same_state(o, e, "kind 4 run")
+    e.engine.sweep(e.counters(2))  # one sweep: the arrays swap once more
+    o.sweeps(1)
+    o.energy()
+    assert np.array_equal(e.spins(), o.spins()), "kind 4 ea_sweep spins"
+    assert np.array_equal(e.engine.energy().ravel(), o.field("ener")), "kind 4 energies"
+    e.engine.set_shape(64, 512, 2, 0)  # another kind continues from the current array
+    e.steps(1, **kw)
+    o.run(1, rows=1, **kw)
+    assert_same_state(o, e, "kind 2 after kind 4")
+
+
+def test_full_size_slab_sweep_bit_exact():
+    """L=48 Lz=384 PBC, 4 replicas, 32 temperatures: one kind-4 sweep
+    (twelve slabs of 8 planes) of three samples against the oracle, then a
+    step with PT and the cluster move."""
+    o, e = pair(48, 384, 4, 32, 3, bc=0, tmin=0.5, tmax=1.6,
+                seeds=(48, 384, 4, 33))
+    e.engine.set_shape(64, 1024, 4, 1)
+    e.engine.sweep(e.counters(2))
+    orc.set_threads(8)
+    try:
+        o.sweeps(1)
+        o.energy()
+        assert np.array_equal(e.spins(), o.spins())
+        assert np.array_equal(e.engine.energy().ravel(), o.field("ener"))
+        e.steps(1, cluster=True)
+        o.steps(1, cluster=True)
+    finally:
+        orc.set_threads(1)
+    assert_same_state(o, e, "large step, kind 4")
+
+
 @pytest.mark.parametrize("ptfr,houfr,mesfr", [(2, 1, 1), (1, 2, 1), (1, 1, 2),
                                               (2, 2, 2)])
 @pytest.mark.parametrize("bc", [0, 1])

@@ -54,7 +54,7 @@ def test_failed_calls_leave_the_engine_unchanged():
     expect(L.ea_sweep(h, 1, None), EA_E_ARG, b"NULL")
     for shape, text in (((48, 1024, 0, 0), b"sweep_positions"),
                         ((64, 128, 0, 0), b"cluster_threads"),
-                        ((64, 1024, 4, 0), b"sweep_kind must be"),
+                        ((64, 1024, 5, 0), b"sweep_kind must be"),
                         ((32, 1024, 1, 0), b"takes sweep_positions 64"),
                         ((64, 1024, 0, 2), b"sweep_xcd")):
         sh = eatube.Shape(*shape)

@@ -172,7 +172,7 @@ def test_shape_signature_and_cache(tmp_path):
     cache = {eatube.shape_signature([a, b]): [32, 512, 0, 1]}
     assert eatube.tune_shape([b, a], cache=cache) == ((32, 512, 0, 1), {})
     assert eatube.DEFAULT_SHAPE in eatube.SHAPES
-    assert len(set(eatube.SHAPES)) == len(eatube.SHAPES) == 30
+    assert len(set(eatube.SHAPES)) == len(eatube.SHAPES) == 36
 
 
 def test_worker_preparation_serves_every_request_once():
