@@ -272,10 +272,16 @@ int ea_batch_run(ea_batch *b, const ea_schedule *schedules, uint64_t *rng_offset
  *   sweep_kind 2: kind 1 with each sample's neighbour words staged in LDS
  *     tiles (sweep_positions must be 64; the tube's tile, 256 + 2*L*L/32
  *     half sites, must fit 768, else EA_E_ARG);
- *   sweep_kind 3: a whole sweep (both halves) per launch, each 1024-thread
+ *   sweep_kind 3: a whole sweep (both halves) per launch, each 512-thread
  *     block holding whole clones of several samples in LDS
  *     (sweep_positions must be 64; tubes with L*L*Lz/128 <= 512, else
  *     EA_E_ARG);
+ *   sweep_kind 4: a whole sweep per launch over z-slabs of the clones,
+ *     the new words written to a second spin array that the engine
+ *     allocates when the kind is set (ea_footprint does not count it) and
+ *     swaps with the first after every sweep (sweep_positions must be 64;
+ *     tubes with a slab thickness T dividing Lz/4, Lz/4 >= T + 4,
+ *     (T + 2)*L*L/32 <= 1024 and (2T + 6)*L*L/32 <= 2048, else EA_E_ARG);
  *   sweep_xcd 1: each XCD runs a contiguous range of sweep blocks;
  *   cluster_threads: threads of a whole-tube Houdayer workgroup (256, 512
  *     or 1024).
