@@ -1067,16 +1067,18 @@ __device__ __forceinline__ uint64_t wrap_down(uint64_t o, uint32_t f, uint32_t a
   return op3<kSelAB>(o >> (f * a), o << (f * b), m);
 }
 __device__ __forceinline__ void fix_flags_sel(uint32_t w, bool top, bool bottom, int parity,
-                                              SiteIn &x) {
+                                              bool obc, SiteIn &x) {
   x.sc[0] = wrap_up(x.sc[0], w & 1, 3, 1, 0x8888888888888888ULL);
   x.sc[3] = wrap_down(x.sc[3], (w >> 3) & 1, 3, 1, 0x1111111111111111ULL);
   x.sc[1] = wrap_up(x.sc[1], (w >> 1) & 1, 12, 4, 0xF000F000F000F000ULL);
   x.sc[4] = wrap_down(x.sc[4], (w >> 4) & 1, 12, 4, 0x000F000F000F000FULL);
   if (w & 4) x.sc[2] = wrap_zp(x.sc[2]);
   if (w & 32) x.sc[5] = wrap_zm(x.sc[5]);
-  const uint64_t aux = parity == 0 ? x.sc[1] : x.sc[4];
-  x.sc[2] = top ? (x.sc[2] & ~kMaskD) | (aux & kMaskD) : x.sc[2];
-  x.sc[5] = bottom ? (x.sc[5] & ~kMaskU) | (aux & kMaskU) : x.sc[5];
+  if (obc) {  // uniform: periodic tubes skip the substitutes
+    const uint64_t aux = parity == 0 ? x.sc[1] : x.sc[4];
+    x.sc[2] = top ? (x.sc[2] & ~kMaskD) | (aux & kMaskD) : x.sc[2];
+    x.sc[5] = bottom ? (x.sc[5] & ~kMaskU) | (aux & kMaskU) : x.sc[5];
+  }
 }
 
 // A copy the compiler cannot see through: per-lane byte offsets made
@@ -1309,7 +1311,7 @@ __global__ void __launch_bounds__(kSlabThreads, 4) k_sweep_slab(const Tube *__re
       const uint32_t ti = ev[i] ? (uint32_t)(tx + i * kSlabThreads) : 0u;
       const u64x2 id = *(__attribute__((address_space(3))) u64x2 *)(lds + thr0 + 16u * ti);
       const uint32_t fl = opaque(ss.own) >> 16;  // derived per sample, not held
-      fix_flags_sel(fl & 63, (fl >> 6) & 1, (fl >> 7) & 1, 0, x);
+      fix_flags_sel(fl & 63, (fl >> 6) & 1, (fl >> 7) & 1, 0, g.bc == EA_OBC, x);
       int e = 0;
       const uint64_t nw = x.sv ^ site_flip(x, id[0], id[1], false, e);
       if (ev[i]) lword(BUF + (ss.own & 0xFFFFu)) = nw;
@@ -1332,7 +1334,7 @@ __global__ void __launch_bounds__(kSlabThreads, 4) k_sweep_slab(const Tube *__re
       const uint32_t ti = (uint32_t)ne + (vo ? (uint32_t)(tx + i * kSlabThreads) : 0u);
       const u64x2 id = *(__attribute__((address_space(3))) u64x2 *)(lds + thr0 + 16u * ti);
       const uint32_t fl = opaque(ss.own) >> 16;
-      fix_flags_sel(fl & 63, (fl >> 6) & 1, (fl >> 7) & 1, 1, x);
+      fix_flags_sel(fl & 63, (fl >> 6) & 1, (fl >> 7) & 1, 1, g.bc == EA_OBC, x);
       int ei = 0;
       const uint64_t nw = x.sv ^ site_flip(x, id[0], id[1], ph.energy, ei);
       if (vo) {

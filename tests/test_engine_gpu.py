@@ -532,6 +532,20 @@ def test_slab_sweeps_match_oracle(L, Lz, nt, ns, bc, ptfr):
     assert_same_state(o, e, "kind 2 after kind 4")
 
 
+def test_slab_sweep_refused_where_no_slab_fits():
+    """An 8x16 tube (4 MSC planes) has no slab thickness whose halo planes
+    stay distinct: sweep kind 4 is refused with the reason, and the engine
+    keeps its shape and state."""
+    o, e = pair(8, 16, 4, 4, 2, seeds=(91, 92, 93, 94))
+    before = e.spins().copy()
+    with pytest.raises(eatube.EngineError, match="slab thickness"):
+        e.engine.set_shape(64, 512, 4, 0)
+    e.steps(1, cluster=True)
+    o.steps(1, cluster=True)
+    assert not np.array_equal(before, e.spins())
+    assert_same_state(o, e, "after the refused shape")
+
+
 def test_full_size_slab_sweep_bit_exact():
     """L=48 Lz=384 PBC, 4 replicas, 32 temperatures: one kind-4 sweep
     (twelve slabs of 8 planes) of three samples against the oracle, then a
