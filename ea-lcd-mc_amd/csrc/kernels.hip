@@ -323,24 +323,6 @@ __device__ __forceinline__ void load_lane(const gword *S, const gword *Jb, uint3
   }
 }
 
-// Neighbours brought to the centre's copy layout (load_neighbours), as
-// selects rather than branches so that no wait sits between two samples'
-// loads.
-__device__ __forceinline__ void fix_flags(int w, bool top, bool bottom, int parity, SiteIn &x) {
-  x.sc[0] = (w & 1) ? wrap_xp(x.sc[0]) : x.sc[0];
-  x.sc[1] = (w & 2) ? wrap_yp(x.sc[1]) : x.sc[1];
-  x.sc[2] = (w & 4) ? wrap_zp(x.sc[2]) : x.sc[2];
-  x.sc[3] = (w & 8) ? wrap_xm(x.sc[3]) : x.sc[3];
-  x.sc[4] = (w & 16) ? wrap_ym(x.sc[4]) : x.sc[4];
-  x.sc[5] = (w & 32) ? wrap_zm(x.sc[5]) : x.sc[5];
-  const uint64_t aux = parity == 0 ? x.sc[1] : x.sc[4];
-  x.sc[2] = top ? (x.sc[2] & ~kMaskD) | (aux & kMaskD) : x.sc[2];
-  x.sc[5] = bottom ? (x.sc[5] & ~kMaskU) | (aux & kMaskU) : x.sc[5];
-}
-__device__ __forceinline__ void fix_lane(const LaneSite &ls, int parity, SiteIn &x) {
-  fix_flags(ls.wrap, ls.top, ls.bottom, parity, x);
-}
-
 // v_bitop3_b32 (gfx950): bit i of the result is bit (a_i << 2 | b_i << 1 |
 // c_i) of an 8-bit table.  Tables are written as the function applied to
 // A = 0xF0, B = 0xCC, C = 0xAA.
@@ -358,6 +340,46 @@ __device__ __forceinline__ uint64_t op3(uint64_t a, uint64_t b, uint64_t c) {
   const uint32_t hi = __builtin_amdgcn_bitop3_b32(
       (uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32), (T & 0xFF));
   return (uint64_t)hi << 32 | lo;
+}
+
+// Neighbours brought to the centre's copy layout (load_neighbours) without
+// selects: each x or y permutation is (a & m) | (b & ~m) of two shifts of
+// the word, and a lane that does not wrap shifts by 0, which leaves the word
+// as it is (both forms, with the shift amounts of each direction, 1/3 for x
+// and 4/12 for y, equal the wrap_* functions bit for bit).  The rare z wraps
+// (first and last MSC plane) and the open-boundary substitutes sit behind
+// branches that almost every wave skips; `obc` may be uniform (the tube's
+// boundary) or per lane (top or bottom).  No branch splits two samples'
+// loads: every caller issues them before the first fix-up.
+constexpr uint32_t kSelAB = (kT3A & kT3C) | (kT3B & ~kT3C & 0xFF);  // (a & c) | (b & ~c)
+__device__ __forceinline__ uint64_t wrap_up(uint64_t o, uint32_t f, uint32_t a, uint32_t b,
+                                            uint64_t m) {
+  return op3<kSelAB>(o << (f * a), o >> (f * b), m);
+}
+__device__ __forceinline__ uint64_t wrap_down(uint64_t o, uint32_t f, uint32_t a, uint32_t b,
+                                              uint64_t m) {
+  return op3<kSelAB>(o >> (f * a), o << (f * b), m);
+}
+__device__ __forceinline__ void fix_flags_sel(uint32_t w, bool top, bool bottom, int parity,
+                                              bool obc, SiteIn &x) {
+  x.sc[0] = wrap_up(x.sc[0], w & 1, 3, 1, 0x8888888888888888ULL);
+  x.sc[3] = wrap_down(x.sc[3], (w >> 3) & 1, 3, 1, 0x1111111111111111ULL);
+  x.sc[1] = wrap_up(x.sc[1], (w >> 1) & 1, 12, 4, 0xF000F000F000F000ULL);
+  x.sc[4] = wrap_down(x.sc[4], (w >> 4) & 1, 12, 4, 0x000F000F000F000FULL);
+  if (w & 4) x.sc[2] = wrap_zp(x.sc[2]);
+  if (w & 32) x.sc[5] = wrap_zm(x.sc[5]);
+  if (obc) {  // uniform: periodic tubes skip the substitutes
+    const uint64_t aux = parity == 0 ? x.sc[1] : x.sc[4];
+    x.sc[2] = top ? (x.sc[2] & ~kMaskD) | (aux & kMaskD) : x.sc[2];
+    x.sc[5] = bottom ? (x.sc[5] & ~kMaskU) | (aux & kMaskU) : x.sc[5];
+  }
+}
+
+__device__ __forceinline__ void fix_flags(int w, bool top, bool bottom, int parity, SiteIn &x) {
+  fix_flags_sel((uint32_t)w, top, bottom, parity, top || bottom, x);
+}
+__device__ __forceinline__ void fix_lane(const LaneSite &ls, int parity, SiteIn &x) {
+  fix_flags(ls.wrap, ls.top, ls.bottom, parity, x);
 }
 
 // The Metropolis flips of one word: (j2 j1) = f0+f1+f2 and (j4 j3) =
@@ -1049,36 +1071,6 @@ __device__ __forceinline__ void slab_words(LW lword, uint32_t buf, const SlabSit
   x.sc[4] = lword(pl + 8 * dym);
   x.sc[2] = lword(own + (uint32_t)czp);
   x.sc[5] = lword(own + (uint32_t)czm);
-}
-
-// The wrap fix-ups of fix_flags without selects: each x or y permutation
-// is (a & m) | (b & ~m) of two shifts of the word, and a lane that does not
-// wrap shifts by 0, which leaves the word as it is.  Both forms, with the
-// shift amounts of each direction (1/3 for x, 4/12 for y), equal the
-// wrap_* functions bit for bit.  The rare z wraps (first and last MSC plane)
-// keep wrap_zp/wrap_zm behind a branch that almost every wave skips.
-constexpr uint32_t kSelAB = (kT3A & kT3C) | (kT3B & ~kT3C & 0xFF);  // (a & c) | (b & ~c)
-__device__ __forceinline__ uint64_t wrap_up(uint64_t o, uint32_t f, uint32_t a, uint32_t b,
-                                            uint64_t m) {
-  return op3<kSelAB>(o << (f * a), o >> (f * b), m);
-}
-__device__ __forceinline__ uint64_t wrap_down(uint64_t o, uint32_t f, uint32_t a, uint32_t b,
-                                              uint64_t m) {
-  return op3<kSelAB>(o >> (f * a), o << (f * b), m);
-}
-__device__ __forceinline__ void fix_flags_sel(uint32_t w, bool top, bool bottom, int parity,
-                                              bool obc, SiteIn &x) {
-  x.sc[0] = wrap_up(x.sc[0], w & 1, 3, 1, 0x8888888888888888ULL);
-  x.sc[3] = wrap_down(x.sc[3], (w >> 3) & 1, 3, 1, 0x1111111111111111ULL);
-  x.sc[1] = wrap_up(x.sc[1], (w >> 1) & 1, 12, 4, 0xF000F000F000F000ULL);
-  x.sc[4] = wrap_down(x.sc[4], (w >> 4) & 1, 12, 4, 0x000F000F000F000FULL);
-  if (w & 4) x.sc[2] = wrap_zp(x.sc[2]);
-  if (w & 32) x.sc[5] = wrap_zm(x.sc[5]);
-  if (obc) {  // uniform: periodic tubes skip the substitutes
-    const uint64_t aux = parity == 0 ? x.sc[1] : x.sc[4];
-    x.sc[2] = top ? (x.sc[2] & ~kMaskD) | (aux & kMaskD) : x.sc[2];
-    x.sc[5] = bottom ? (x.sc[5] & ~kMaskU) | (aux & kMaskU) : x.sc[5];
-  }
 }
 
 // A copy the compiler cannot see through: per-lane byte offsets made
