@@ -1139,9 +1139,10 @@ __global__ void __launch_bounds__(kSlabThreads, 4) k_sweep_slab(const Tube *__re
     cw[2] = c->z;
     cw[3] = c->w;
   }
-  // the thread's chunks of a sample buffer: LDS byte offset and word offset
-  // within a sample's spins (parity array, then the clone row's plane)
-  uint32_t cl[kSlabChunks], cg[kSlabChunks];
+  // the thread's chunks of a sample buffer (chunk c fills LDS bytes 16c ..
+  // 16c + 15): word offset within a sample's spins (parity array, then the
+  // clone row's plane)
+  uint32_t cg[kSlabChunks];
 #pragma unroll
   for (int k = 0; k < kSlabChunks; ++k) {
     const int c = min(tx + k * kSlabThreads, nch - 1), w = 2 * c;
@@ -1151,7 +1152,6 @@ __global__ void __launch_bounds__(kSlabThreads, 4) k_sweep_slab(const Tube *__re
     int gz = z0 + lp - (odd ? 2 : 1);
     gz += gz < 0 ? g.MLz : 0;
     gz -= gz >= g.MLz ? g.MLz : 0;
-    cl[k] = 8u * (uint32_t)w;
     cg[k] = (uint32_t)(odd ? g.half : 0) + (uint32_t)(gz * g.hs + off);
   }
   {
@@ -1185,13 +1185,16 @@ __global__ void __launch_bounds__(kSlabThreads, 4) k_sweep_slab(const Tube *__re
   };
   auto stage = [&](uint32_t buf) {
 #pragma unroll
-    for (int k = 0; k < kSlabChunks; ++k)
-      *(__attribute__((address_space(3))) u64x2 *)(lds + buf + cl[k]) = tv[k];
+    for (int k = 0; k < kSlabChunks; ++k) {
+      const uint32_t c = (uint32_t)min(tx + k * kSlabThreads, nch - 1);  // chunk k's LDS slot
+      *(__attribute__((address_space(3))) u64x2 *)(lds + buf + 16u * c) = tv[k];
+    }
   };
   fetch(0);  // in flight while the thresholds are made
   // thresholds of positions pos = tx, tx + kSlabThreads, ...: even ones
   // (pos < ne) with the even half's counter block, odd ones with the odd's
-  const uint32_t thr0 = 2 * kSlabBuf;
+  const uint32_t bstride = 8u * (uint32_t)sweep_slab_words(g, T);  // one sample buffer
+  const uint32_t thr0 = 2 * bstride;
   for (int pos = tx; pos < ne + no; pos += kSlabThreads) {
     const int odd = pos >= ne, q = odd ? pos - ne : pos;
     const int lp = q / g.hs, rem = q - lp * g.hs;
@@ -1268,8 +1271,7 @@ __global__ void __launch_bounds__(kSlabThreads, 4) k_sweep_slab(const Tube *__re
   };
   load_even(0);
   // one sample, its words in the buffer at LDS byte offset BUF
-  auto sample = [&](int s, auto bufc) {
-    constexpr uint32_t BUF = decltype(bufc)::value;
+  auto sample = [&](int s, uint32_t BUF) {
     fetch(min(s + 1, last));  // the next sample's words (after the last: a repeat)
     // uniform bases and 32-bit byte offsets (within a sample's spins or
     // couplings), so that the loads and stores take a scalar base
@@ -1338,17 +1340,14 @@ __global__ void __launch_bounds__(kSlabThreads, 4) k_sweep_slab(const Tube *__re
       e = group_scan<64>(e);
       if ((tx & 63) == 63) atomicAdd(&esum[s & 1], e);
     }
-    stage(BUF ^ kSlabBuf);  // the next sample's words, into the other buffer
+    stage(bstride - BUF);  // the next sample's words, into the other buffer
     lds_barrier();
     if (ph.energy && tx == 0) {
       gv(a.part)[(uint32_t)s * pstride + pbase] = esum[s & 1];
       esum[s & 1] = 0;
     }
   };
-  for (int s = 0; s < g.ns; s += 2) {
-    sample(s, std::integral_constant<uint32_t, 0>());
-    if (s + 1 < g.ns) sample(s + 1, std::integral_constant<uint32_t, kSlabBuf>());
-  }
+  for (int s = 0; s < g.ns; ++s) sample(s, (uint32_t)(s & 1) * bstride);
 }
 
 // After a kind-4 sweep: the tubes that ran it swap their two spin arrays.

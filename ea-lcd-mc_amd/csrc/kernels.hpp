@@ -211,15 +211,14 @@ inline size_t sweep_full_thr_words(const Lattice &g) {
 // kSlabLdsMax; 0 when no T does (the kind does not fit the tube).
 constexpr int kSlabThreads = 512;
 constexpr int kSlabPos = 2;
-constexpr int kSlabChunks = 2;        // 16-byte chunks per thread and sample
-constexpr uint32_t kSlabBuf = 16384;  // LDS bytes of one sample's buffer
+constexpr int kSlabChunks = 3;             // 16-byte chunks per thread and sample
 constexpr size_t kSlabLdsMax = 69 * 1024;  // dynamic LDS: two blocks per CU
 __host__ __device__ inline int sweep_slab_words(const Lattice &g, int T) {
   return (2 * T + 6) * g.hs;  // one sample's buffer: odd then even planes
 }
 __host__ __device__ inline size_t sweep_slab_lds(const Lattice &g, int T) {
   // two sample buffers, then (id1, id2) of the (2T + 2) hs positions
-  return 2 * (size_t)kSlabBuf + 16 * (size_t)(2 * T + 2) * g.hs;
+  return 16 * (size_t)sweep_slab_words(g, T) + 16 * (size_t)(2 * T + 2) * g.hs;
 }
 inline int sweep_slab_planes(const Lattice &g) {
   if (16 * g.half >= (1LL << 32)) return 0;  // 32-bit byte offsets within a sample
@@ -229,7 +228,7 @@ inline int sweep_slab_planes(const Lattice &g) {
   for (int T = g.MLz; T >= 2; --T) {
     if (g.MLz % T || g.MLz < T + 4 || g.MLz / T > slots) continue;
     if ((T + 2) * g.hs > kSlabPos * kSlabThreads) continue;
-    if (8 * (size_t)sweep_slab_words(g, T) > kSlabBuf) continue;
+    if (sweep_slab_words(g, T) > 2 * kSlabChunks * kSlabThreads) continue;
     if (sweep_slab_lds(g, T) > kSlabLdsMax) continue;
     return T;
   }

@@ -513,8 +513,8 @@ def test_slab_sweeps_match_oracle(L, Lz, nt, ns, bc, ptfr):
     oracle, through ea_run with cluster moves and through ea_sweep with an
     odd and then an even number of sweeps: slabs of 8 planes (two of them,
     each the other's halo across the periodic ends), of 2 planes (three,
-    open boundaries), three slabs of 8 planes at L=48 and two at L=32; odd
-    sample counts end the sample loop in the first LDS buffer."""
+    open boundaries), two slabs of 12 planes at L=48 and two of 8 at L=32;
+    odd sample counts end the sample loop in the first LDS buffer."""
     o, e = pair(L, Lz, 4, nt, ns, bc=bc, seeds=(81, 82, 83, 84 + bc))
     e.engine.set_shape(64, 512, 4, 1)
     kw = dict(ptfr=ptfr, cluster=True)
@@ -548,7 +548,7 @@ def test_slab_sweep_refused_where_no_slab_fits():
 
 def test_full_size_slab_sweep_bit_exact():
     """L=48 Lz=384 PBC, 4 replicas, 32 temperatures: one kind-4 sweep
-    (twelve slabs of 8 planes) of three samples against the oracle, then a
+    (eight slabs of 12 planes) of three samples against the oracle, then a
     step with PT and the cluster move."""
     o, e = pair(48, 384, 4, 32, 3, bc=0, tmin=0.5, tmax=1.6,
                 seeds=(48, 384, 4, 33))
