@@ -893,7 +893,7 @@ static int check_shape(const ea_shape *sh) {
 static const char *shape_misfit(int kind) {
   if (kind == 4)
     return "no slab thickness T divides Lz/4 with Lz/4 >= T + 4, (T + 2) L*L/32 <= 1024 "
-           "and (2T + 6) L*L/32 <= 2048";
+           "and (2T + 6) L*L/32 <= 3072";
   return kind == 3 ? "the clones of a one-launch sweep need n = L*L*Lz/128 <= 512"
                    : "the tile (256 + L*L/16 half sites) exceeds 768";
 }

@@ -281,7 +281,8 @@ int ea_batch_run(ea_batch *b, const ea_schedule *schedules, uint64_t *rng_offset
  *     allocates when the kind is set (ea_footprint does not count it) and
  *     swaps with the first after every sweep (sweep_positions must be 64;
  *     tubes with a slab thickness T dividing Lz/4, Lz/4 >= T + 4,
- *     (T + 2)*L*L/32 <= 1024 and (2T + 6)*L*L/32 <= 2048, else EA_E_ARG);
+ *     (T + 2)*L*L/32 <= 1024 and (2T + 6)*L*L/32 <= 3072 words in LDS,
+ *     else EA_E_ARG);
  *   sweep_xcd 1: each XCD runs a contiguous range of sweep blocks;
  *   cluster_threads: threads of a whole-tube Houdayer workgroup (256, 512
  *     or 1024).
